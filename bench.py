@@ -1,0 +1,146 @@
+"""Benchmark of the plain FP64 step: Orszag-Tang 4096^2 (unit box), HLLD, van Leer.
+
+One step = compute_dt (device max-reduction fused into the previous y sweep, 8-byte
+read-back, host dt/c_h arithmetic) + advance (x sweep, ghost fill, y sweep with fused
+psi damping and CFL candidate, ghost fill).  The state is resident in HBM before the
+timed region starts.  Prints ONE JSON line.
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--n 4096] [--solver hlld]
+                    [--limiter vanleer] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+BYTES_PER_CELL_SWEEP = 144  # 9 FP64 read + 9 FP64 written per cell per directional sweep
+BYTES_PER_CELL_STEP = 288   # two sweeps
+SOLVERS = {"hll": 0, "hllc": 1, "hlld": 2, "force": 3}
+LIMITERS = {"minmod": 0, "vanleer": 1}
+KERNELS = ("xsweep", "bc_after_x", "ysweep", "bc_after_y")
+
+
+def read_traffic(n, solver, limiter):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled
+    by a copy-kernel calibration), from profiles/pmc_*.txt when present."""
+    path = os.path.join(ROOT, "profiles", f"pmc_ot{n}_{solver}_{limiter}.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.strip().split("=", 1)
+            vals[k.strip()] = v.strip()
+    return vals
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--solver", default="hlld", choices=sorted(SOLVERS))
+    ap.add_argument("--limiter", default="vanleer", choices=sorted(LIMITERS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; the path shards by y-slabs but no multi-GPU "
+              "run is part of it (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import mhd_gpu_acc_amd as m
+
+    torch.cuda.init()
+    n = args.n
+    solver, limiter = SOLVERS[args.solver], LIMITERS[args.limiter]
+    cfg = m.get_n_case_config("orszag_tang_unit", 1)
+    cfg.nx = cfg.ny = n
+    U0 = m.initial_state(m.CaseId.OrszagTangUnit, cfg)
+    cells = n * n
+    run = m.Run(cfg, solver, limiter)
+    run.set_state(U0)
+    L = m.lib()
+    import ctypes as C
+
+    # warm-up (also the first steps of the parity sample)
+    w = run.step_to(cfg.t_end, args.warmup)
+    assert w["code"] == 0, w
+    L.mhd_profile(run.handle, 1)
+    torch.cuda.synchronize()
+    run.synchronize()
+    t0 = time.perf_counter()
+    r = run.step_to(cfg.t_end, args.steps)
+    run.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    assert r["code"] == 0 and r["steps"] == args.steps, r
+    ms4 = (C.c_double * 4)()
+    nst = C.c_int()
+    L.mhd_profile_read(run.handle, ms4, C.byref(nst))
+    L.mhd_profile(run.handle, 0)
+    elapsed = t1 - t0
+    value = args.steps * cells / elapsed
+    kern_ms = {k: ms4[i] / max(nst.value, 1) for i, k in enumerate(KERNELS)}
+    dom = max(("xsweep", "ysweep"), key=lambda k: kern_ms[k])
+    achieved = BYTES_PER_CELL_SWEEP * cells / (kern_ms[dom] * 1e-3)
+    traffic = read_traffic(n, args.solver, args.limiter)
+    gpu_dts = np.concatenate([w["dts"], r["dts"]])
+    run.close()
+
+    out = {
+        "metric": "cell-updates/s + %HBM peak, OT 4096² HLLD FP64; batched & adjoint vs plain",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: analytic Orszag-Tang initial state (no dataset)",
+        "config": {"workload": f"orszag_tang_unit {n}x{n} periodic, {args.solver.upper()}, {args.limiter}, "
+                               f"CFL {cfg.cfl}, plain step (compute_dt + advance)",
+                   "nx": n, "ny": n, "solver": args.solver, "limiter": args.limiter, "cfl": cfg.cfl},
+        "hbm_frac_step": value * BYTES_PER_CELL_STEP / HBM_PEAK,
+        "kernel_ms": kern_ms,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": (float(traffic["bytes_per_launch_" + dom]) if traffic and
+                                 ("bytes_per_launch_" + dom) in traffic else None),
+                     "algorithmic_bytes_per_launch": BYTES_PER_CELL_SWEEP * cells},
+    }
+
+    if not args.no_cpu:
+        import oracle
+        case = oracle.Case(n, n, 2, cfg.x_min, cfg.x_max, cfg.y_min, cfg.y_max, cfg.cfl, cfg.t_end, cfg.gamma,
+                           tuple(cfg.bc.as_list()))
+        _, t_one = oracle.run_loop_timed(U0, case, solver, limiter, 1)
+        k = int(max(1, min(40, round(args.cpu_seconds / max(t_one, 1e-9)))))
+        res, secs = oracle.run_loop_timed(U0, case, solver, limiter, k)
+        kk = min(k, len(gpu_dts))
+        same = bool(np.array_equal(res["dts"][:kk].view(np.uint64), gpu_dts[:kk].view(np.uint64)))
+        out["cpu_baseline"] = {"value": k * cells / secs, "unit": "cell-updates/s",
+                               "cores": oracle.num_threads(), "kind": "port",
+                               "sample": f"first {k} steps of the same {n}^2 run ({secs:.1f} s), "
+                                         f"C++/OpenMP restatement of the reference CPU path"}
+        out["parity"] = {"dt_sequence_bitwise_vs_cpu": same, "steps_compared": kk}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

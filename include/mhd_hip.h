@@ -1,0 +1,167 @@
+/*
+ * mhd_hip.h -- C ABI of the MI355X (gfx950) GLM-MHD MUSCL-Hancock step.
+ *
+ * This is the drop-in boundary for the reference's solver interface.  Each
+ * entry point names the reference interface it replaces.  No C++ types, no
+ * exceptions and no torch types cross it: plain pointers, sizes, enums as
+ * ints and an int status (MHD_OK == 0); mhd_last_error() returns the message
+ * of the last failure on the calling thread.
+ *
+ * State layout.  A grid is 9 SoA planes of float64 in the reference field
+ * order (rho, rho*u, rho*v, rho*w, Bx, By, Bz, E, psi) -- head/types.hpp:16-26.
+ * Each plane holds (nx + 2 ng) x (ny + 2 ng) cells, row-major, ghost cells
+ * included, index j * (nx + 2 ng) + i -- head/cpu/grid_cpu.hpp:85-87,
+ * head/gpu/grid_gpu.cuh:102-110.  ng is 2.  Planes are `plane_stride`
+ * doubles apart (mhd_plane_stride()).
+ */
+#ifndef MHD_HIP_H
+#define MHD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------- */
+enum {
+    MHD_OK = 0,
+    MHD_ERR_INVALID = 1,    /* bad argument, size mismatch, unknown enum */
+    MHD_ERR_HIP = 2,        /* HIP runtime error (allocation, launch, copy) */
+    MHD_ERR_NO_SIGNAL = 3,  /* compute_dt: non-positive maximum wave speed
+                               (solver_cpu.cpp:304-306 throws here) */
+    MHD_ERR_DT = 4,         /* step_to: dt non-finite or <= 0 (main_cpu.cpp:205-209) */
+    MHD_ERR_DT_FLOOR = 5,   /* step_to: dt < 1e-8 t_end, blow-up stop (main_cpu.cpp:210-216) */
+    MHD_ERR_NOT_BUILT = 6   /* no HIP device or kernels unavailable */
+};
+
+/* ---- enums: same numeric order as the reference -------------------------------- */
+/* RiemannSolver { HLL, HLLC, HLLD, FORCE } -- head/riemann.hpp:23 */
+enum { MHD_HLL = 0, MHD_HLLC = 1, MHD_HLLD = 2, MHD_FORCE = 3 };
+/* BoundaryType { Periodic, Transmissive, Reflecting, Dirichlet } -- head/test_cases.hpp:11-16 */
+enum { MHD_BC_PERIODIC = 0, MHD_BC_TRANSMISSIVE = 1, MHD_BC_REFLECTING = 2, MHD_BC_DIRICHLET = 3 };
+/* Slope limiter: minmod is the reference's (solver_cpu.cpp:26-43); van Leer is new. */
+enum { MHD_LIMITER_MINMOD = 0, MHD_LIMITER_VANLEER = 1 };
+/* CaseId { KelvinHelmholtz, ShockBubble, BrioWu, OrszagTang } -- head/test_cases.hpp:29-34,
+ * then the cases this build adds. */
+enum {
+    MHD_CASE_KELVIN_HELMHOLTZ = 0,
+    MHD_CASE_SHOCK_BUBBLE = 1,
+    MHD_CASE_BRIO_WU = 2,
+    MHD_CASE_ORSZAG_TANG = 3,
+    MHD_CASE_ORSZAG_TANG_UNIT = 4,  /* [0,1]^2, rho = 25/(36 pi), p = 5/(12 pi), B / sqrt(4 pi) */
+    MHD_CASE_SHOCK_CLOUD = 5,       /* [0,2]x[0,1] MHD shock-cloud with seeded bubble noise */
+    MHD_CASE_BRIO_WU_T = 6,         /* Brio-Wu, transmissive in y, CFL 0.8 */
+    MHD_CASE_OT_FAMILY = 7,         /* batch OT family: gamma, B scale, 4-mode density ripple */
+    MHD_CASE_OT_BUMP = 8            /* unit OT plus a Gaussian density bump */
+};
+
+/* Per-run description (replaces the process-global phys::gamma / ch_glm / cr_glm of
+ * head/physics.hpp:26-43 and the GpuWorkspace of head/gpu/solver_gpu.cuh:9-20). */
+typedef struct {
+    int nx, ny, ng;                      /* ng must be 2 */
+    double x_min, x_max, y_min, y_max;   /* dx = (x_max - x_min) / nx, grid_cpu.hpp:25-26 */
+    double gamma;                        /* adiabatic index */
+    double cr;                           /* mixed-GLM c_r = c_p^2 / c_h (0.18) */
+    double cfl;
+    int solver;                          /* MHD_HLL .. MHD_FORCE */
+    int limiter;                         /* MHD_LIMITER_* */
+    int bc[4];                           /* left, right, bottom, top */
+    void* stream;                        /* hipStream_t; NULL = the null stream */
+} mhd_run_desc;
+
+/* Case table entry (head/test_cases.hpp:38-55 CaseConfig). */
+typedef struct {
+    int nx, ny, ng;
+    double x_min, x_max, y_min, y_max;
+    double cfl, t_end, gamma;
+    int bc[4];
+} mhd_case_config;
+
+/* Parameters of the analytic ICs this build adds (ignored by the reference cases). */
+typedef struct {
+    double gamma;
+    double b_scale;
+    uint64_t seed;
+    int n_modes;
+    double mode_amp[4], mode_kx[4], mode_ky[4], mode_phase[4];
+    double pert_amp;
+    double bump_amp, bump_sigma, bump_cx, bump_cy;
+} mhd_ic_params;
+
+typedef struct mhd_run mhd_run;
+
+const char* mhd_last_error(void);
+int mhd_device_count(int* n);
+
+/* ---- problem set-up: src/test_cases.cpp:178-300, src/init.cpp:18-65 ---------- */
+/* get_n_case_config(CaseId, n) -- test_cases.cpp:275-282 */
+int mhd_case_config_get(int case_id, int n_scale, mhd_case_config* out);
+/* initialise_grid -- init.cpp:34-43: fills every cell, ghosts included, of a host
+ * AoS array (cells x 9, reference field order) from the analytic IC. */
+int mhd_init_state_host(int case_id, const mhd_case_config* geom, const mhd_ic_params* params, double* aos);
+
+/* ---- run handle: Grid2DGPU x 3 + GpuWorkspace (grid_gpu.cuh:17-223,
+ *      solver_gpu.cuh:9-35): allocates Uold/Utmp/Unew on the device. ------------ */
+int mhd_run_create(const mhd_run_desc* desc, mhd_run** out);
+int mhd_run_destroy(mhd_run* run);
+size_t mhd_plane_stride(const mhd_run* run);   /* doubles between planes */
+/* Device base pointer of the current state (9 planes, plane_stride apart). */
+double* mhd_state_ptr(mhd_run* run);
+
+/* upload_from_aos / download_to_aos -- grid_gpu.cuh:120-177.
+ * layout 0: host AoS (cells x 9); layout 1: host SoA (9 x cells, dense);
+ * layout 2: device SoA (9 x cells, dense). */
+int mhd_set_state(mhd_run* run, const double* src, int layout);
+int mhd_get_state(mhd_run* run, double* dst, int layout);
+
+/* set_gpu_physics_ch / the host mirror phys::ch_glm -- solver_gpu.cu:536-544 */
+int mhd_set_ch(mhd_run* run, double ch);
+int mhd_get_ch(const mhd_run* run, double* ch);
+
+/* compute_dt(const Grid2D&, cfl) -- solver_cpu.hpp:49, solver_cpu.cpp:283-312;
+ * compute_dt_gpu -- solver_gpu.cuh:40.  Ratchets c_h to max(c_h, max signal
+ * speed) and returns dt = cfl min(dx, dy) / c_h.  MHD_ERR_NO_SIGNAL where the
+ * reference throws.  Synchronises the run's stream. */
+int mhd_compute_dt(mhd_run* run, double* dt);
+
+/* advance_second_order(Uold, Utmp, Unew, dt, ws, solver, bc) -- solver_cpu.cpp:327-440,
+ * advance_second_order_gpu -- solver_gpu.cuh:45-53.  x-sweep, BC, y-sweep, BC,
+ * psi damping; then Uold <- Unew.  Asynchronous on the run's stream. */
+int mhd_advance(mhd_run* run, double dt);
+
+/* The driver loop of scripts/cpu/main_cpu.cpp:196-222 without output:
+ * while (t < t_end && steps < max_steps) { dt = min(compute_dt, t_end - t);
+ * guards; advance; t += dt }.  dts/chs (nullable, max_steps long) receive
+ * each step's dt and c_h.  Returns MHD_OK, MHD_ERR_NO_SIGNAL, MHD_ERR_DT or
+ * MHD_ERR_DT_FLOOR; *steps and *t always hold what was done. */
+int mhd_step_to(mhd_run* run, double t_end, int max_steps, double* t, int* steps, double* dts, double* chs);
+
+/* apply_boundary_gpu -- boundary_gpu.cuh:9 (boundary_cpu.hpp:45-149): BC on the
+ * current state, in place (Dirichlet sides keep their values). */
+int mhd_apply_boundary(mhd_run* run);
+
+/* ---- grid-level entry points over caller-owned device grids ------------------------
+ * Each grid is 9 dense SoA planes of (nx+4)(ny+4) doubles (plane stride = cells).
+ * These are the Grid2DGPU-view forms of compute_dt_gpu / advance_second_order_gpu;
+ * the run only supplies parameters, c_h and scratch. */
+int mhd_compute_dt_grid(mhd_run* run, const double* U, double* dt);
+int mhd_advance_grids(mhd_run* run, const double* Uold, double* Utmp, double* Unew, double dt);
+
+/* Synchronise the run's stream (errors from asynchronous launches surface here). */
+int mhd_sync(mhd_run* run);
+
+/* ---- measurement (no counterpart in the reference, whose only timer is the
+ *      steady_clock around the loop, main_cpu.cpp:191,254-269) --------------------
+ * While enabled, every step records HIP events on the run's stream around its
+ * launches.  mhd_profile_read synchronises and returns the summed milliseconds of
+ * each launch kind since enabling -- ms[0] x sweep, ms[1] ghost fill after it,
+ * ms[2] y sweep, ms[3] ghost fill after it -- and the number of steps. */
+int mhd_profile(mhd_run* run, int enable);
+int mhd_profile_read(mhd_run* run, double* ms4, int* steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MHD_HIP_H */

@@ -1,0 +1,340 @@
+// Host layer of the C ABI (include/mhd_hip.h): run handles, state transfer,
+// dt control and the driver loop, over the kernels of kernels.hip.
+//
+// The time-step control keeps the reference's host arithmetic: c_h ratchets to
+// the largest signal speed (solver_cpu.cpp:283-312), dt = (cfl min(dx,dy)) / c_h,
+// and the psi damping factor is exp(((-dt) c_h) / c_r) evaluated with the host
+// libm (solver_cpu.cpp:264).  The maximum signal speed is reduced on the device
+// (fused into the y sweep of the previous step, or by the stand-alone CFL kernel)
+// and read back as 8 bytes per step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mhd_hip.h"
+#include "launch.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+int hip_fail(hipError_t e, const char* what) {
+    return fail(MHD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define MHD_HIP(call)                                  \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return hip_fail(e_, #call); \
+    } while (0)
+
+}  // namespace
+
+struct mhd_run {
+    mhd_run_desc d;
+    int tnx, tny;
+    size_t cells;  // (nx+4)(ny+4): plane stride of every grid
+    double dx, dy;
+    double* U[3] = {nullptr, nullptr, nullptr};  // current, intermediate (U*), next
+    double* frame = nullptr;
+    unsigned long long* acc = nullptr;
+    unsigned long long* host_acc = nullptr;  // pinned
+    double ch = 0.0;                         // c_h, starts at 0 (physics.hpp:41)
+    bool acc_valid = false;                  // acc holds the CFL max of U[0] with the current ch
+    int seg = 0;
+    bool profiling = false;
+    std::vector<hipEvent_t> events;  // 5 per profiled step
+    int prof_steps = 0;
+    hipStream_t stream() const { return static_cast<hipStream_t>(d.stream); }
+};
+
+namespace {
+
+int check_desc(const mhd_run_desc* d) {
+    if (!d) return fail(MHD_ERR_INVALID, "null run description");
+    if (d->ng != 2) return fail(MHD_ERR_INVALID, "ng must be 2");
+    if (d->nx < 2 || d->ny < 2) return fail(MHD_ERR_INVALID, "nx and ny must be >= 2");
+    if (d->solver < 0 || d->solver > 3) return fail(MHD_ERR_INVALID, "unknown Riemann solver");
+    if (d->limiter < 0 || d->limiter > 1) return fail(MHD_ERR_INVALID, "unknown limiter");
+    for (int k = 0; k < 4; ++k)
+        if (d->bc[k] < 0 || d->bc[k] > 3) return fail(MHD_ERR_INVALID, "unknown boundary type");
+    if (!(d->x_max > d->x_min) || !(d->y_max > d->y_min)) return fail(MHD_ERR_INVALID, "empty domain");
+    return MHD_OK;
+}
+
+// Host read of the device CFL maximum (a non-negative double as raw bits).
+int read_acc(mhd_run* r, double* out) {
+    MHD_HIP(hipMemcpyAsync(r->host_acc, r->acc, sizeof(unsigned long long), hipMemcpyDeviceToHost, r->stream()));
+    MHD_HIP(hipStreamSynchronize(r->stream()));
+    unsigned long long bits = *r->host_acc;
+    std::memcpy(out, &bits, sizeof bits);
+    return MHD_OK;
+}
+
+// compute_dt on a grid: reduction on the device, ratchet and dt on the host.
+int dt_from_state(mhd_run* r, const double* U, bool reuse, double* dt) {
+    if (!reuse) {
+        MHD_HIP(hipMemsetAsync(r->acc, 0, sizeof(unsigned long long), r->stream()));
+        MHD_HIP(mhd::launch_cfl(U, r->d.nx, r->d.ny, (long long)r->cells, r->d.gamma, r->ch, r->acc, r->stream()));
+    }
+    double mx = 0.0;
+    if (int rc = read_acc(r, &mx)) return rc;
+    if (reuse == false && U == r->U[0]) r->acc_valid = true;
+    if (!(mx > 0.0)) return fail(MHD_ERR_NO_SIGNAL, "compute_dt: non-positive maximum wave speed.");
+    r->ch = mx;
+    *dt = r->d.cfl * std::min(r->dx, r->dy) / mx;
+    return MHD_OK;
+}
+
+int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt) {
+    mhd::StepArgs a{};
+    a.Uold = Uold;
+    a.Utmp = Utmp;
+    a.Unew = Unew;
+    a.nx = r->d.nx;
+    a.ny = r->d.ny;
+    a.ps = (long long)r->cells;
+    a.dx = r->dx;
+    a.dy = r->dy;
+    a.dt = dt;
+    a.gamma = r->d.gamma;
+    a.ch = r->ch;
+    a.damp = std::exp(-dt * r->ch / r->d.cr);
+    a.solver = r->d.solver;
+    a.limiter = r->d.limiter;
+    std::memcpy(a.bc, r->d.bc, sizeof a.bc);
+    a.seg = r->seg;
+    a.frame = r->frame;
+    a.acc = r->acc;
+    a.stream = r->stream();
+    a.ev = nullptr;
+    if (r->profiling) {
+        const size_t base = r->events.size();
+        for (int k = 0; k < 5; ++k) {
+            hipEvent_t ev;
+            MHD_HIP(hipEventCreate(&ev));
+            r->events.push_back(ev);
+        }
+        a.ev = r->events.data() + base;
+        r->prof_steps += 1;
+    }
+    MHD_HIP(mhd::launch_step(a));
+    return MHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mhd_last_error(void) { return g_err.c_str(); }
+
+int mhd_device_count(int* n) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+    return MHD_OK;
+}
+
+int mhd_run_create(const mhd_run_desc* desc, mhd_run** out) {
+    if (!out) return fail(MHD_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    if (int rc = check_desc(desc)) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(MHD_ERR_NOT_BUILT, "no HIP device");
+    mhd_run* r = new mhd_run();
+    r->d = *desc;
+    r->tnx = desc->nx + 4;
+    r->tny = desc->ny + 4;
+    r->cells = (size_t)r->tnx * r->tny;
+    r->dx = (desc->x_max - desc->x_min) / static_cast<double>(desc->nx);  // grid_cpu.hpp:25-26
+    r->dy = (desc->y_max - desc->y_min) / static_cast<double>(desc->ny);
+    const char* seg = std::getenv("MHD_YSEG");
+    r->seg = seg ? std::atoi(seg) : 0;
+    auto cleanup = [&](hipError_t e, const char* what) {
+        mhd_run_destroy(r);
+        return hip_fail(e, what);
+    };
+    for (int k = 0; k < 3; ++k) {
+        hipError_t e = hipMalloc(&r->U[k], 9 * r->cells * sizeof(double));
+        if (e != hipSuccess) return cleanup(e, "hipMalloc(grid)");
+        e = hipMemsetAsync(r->U[k], 0, 9 * r->cells * sizeof(double), r->stream());
+        if (e != hipSuccess) return cleanup(e, "hipMemset(grid)");
+    }
+    hipError_t e = hipMalloc(&r->frame, 4 * (size_t)(desc->nx + desc->ny) * sizeof(double));
+    if (e != hipSuccess) return cleanup(e, "hipMalloc(frame)");
+    if ((e = hipMalloc(&r->acc, sizeof(unsigned long long))) != hipSuccess) return cleanup(e, "hipMalloc(acc)");
+    if ((e = hipHostMalloc(&r->host_acc, sizeof(unsigned long long), hipHostMallocDefault)) != hipSuccess)
+        return cleanup(e, "hipHostMalloc");
+    if ((e = hipStreamSynchronize(r->stream())) != hipSuccess) return cleanup(e, "hipStreamSynchronize");
+    *out = r;
+    return MHD_OK;
+}
+
+int mhd_run_destroy(mhd_run* r) {
+    if (!r) return MHD_OK;
+    hipStreamSynchronize(r->stream());
+    for (hipEvent_t ev : r->events) hipEventDestroy(ev);
+    for (double* p : r->U)
+        if (p) hipFree(p);
+    if (r->frame) hipFree(r->frame);
+    if (r->acc) hipFree(r->acc);
+    if (r->host_acc) hipHostFree(r->host_acc);
+    delete r;
+    return MHD_OK;
+}
+
+size_t mhd_plane_stride(const mhd_run* r) { return r ? r->cells : 0; }
+double* mhd_state_ptr(mhd_run* r) { return r ? r->U[0] : nullptr; }
+
+int mhd_set_state(mhd_run* r, const double* src, int layout) {
+    if (!r || !src) return fail(MHD_ERR_INVALID, "null argument");
+    const size_t n = r->cells;
+    if (layout == 0) {
+        std::vector<double> soa(9 * n);
+        for (size_t c = 0; c < n; ++c)
+            for (int f = 0; f < 9; ++f) soa[f * n + c] = src[9 * c + f];
+        MHD_HIP(hipMemcpy(r->U[0], soa.data(), 9 * n * sizeof(double), hipMemcpyHostToDevice));
+    } else if (layout == 1) {
+        MHD_HIP(hipMemcpy(r->U[0], src, 9 * n * sizeof(double), hipMemcpyHostToDevice));
+    } else if (layout == 2) {
+        MHD_HIP(hipMemcpyAsync(r->U[0], src, 9 * n * sizeof(double), hipMemcpyDeviceToDevice, r->stream()));
+    } else {
+        return fail(MHD_ERR_INVALID, "unknown layout");
+    }
+    // Utmp/Unew start as copies (scripts/cpu/main_cpu.cpp:173-174); only their
+    // ghost cells are ever read before being written (Dirichlet sides).
+    MHD_HIP(hipMemcpyAsync(r->U[1], r->U[0], 9 * n * sizeof(double), hipMemcpyDeviceToDevice, r->stream()));
+    MHD_HIP(hipMemcpyAsync(r->U[2], r->U[0], 9 * n * sizeof(double), hipMemcpyDeviceToDevice, r->stream()));
+    MHD_HIP(hipStreamSynchronize(r->stream()));
+    r->acc_valid = false;
+    return MHD_OK;
+}
+
+int mhd_get_state(mhd_run* r, double* dst, int layout) {
+    if (!r || !dst) return fail(MHD_ERR_INVALID, "null argument");
+    const size_t n = r->cells;
+    MHD_HIP(hipStreamSynchronize(r->stream()));
+    if (layout == 0) {
+        std::vector<double> soa(9 * n);
+        MHD_HIP(hipMemcpy(soa.data(), r->U[0], 9 * n * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t c = 0; c < n; ++c)
+            for (int f = 0; f < 9; ++f) dst[9 * c + f] = soa[f * n + c];
+    } else if (layout == 1) {
+        MHD_HIP(hipMemcpy(dst, r->U[0], 9 * n * sizeof(double), hipMemcpyDeviceToHost));
+    } else if (layout == 2) {
+        MHD_HIP(hipMemcpyAsync(dst, r->U[0], 9 * n * sizeof(double), hipMemcpyDeviceToDevice, r->stream()));
+        MHD_HIP(hipStreamSynchronize(r->stream()));
+    } else {
+        return fail(MHD_ERR_INVALID, "unknown layout");
+    }
+    return MHD_OK;
+}
+
+int mhd_set_ch(mhd_run* r, double ch) {
+    if (!r) return fail(MHD_ERR_INVALID, "null run");
+    r->ch = ch;
+    r->acc_valid = false;
+    return MHD_OK;
+}
+
+int mhd_get_ch(const mhd_run* r, double* ch) {
+    if (!r || !ch) return fail(MHD_ERR_INVALID, "null argument");
+    *ch = r->ch;
+    return MHD_OK;
+}
+
+int mhd_compute_dt(mhd_run* r, double* dt) {
+    if (!r || !dt) return fail(MHD_ERR_INVALID, "null argument");
+    // After a step the y sweep has already reduced max(c_h, signal speed) over the
+    // new state with the c_h a fresh compute_dt would use; a repeated call is
+    // idempotent (max over cells of fmax(s, M) is M once c_h = M).
+    return dt_from_state(r, r->U[0], r->acc_valid, dt);
+}
+
+int mhd_advance(mhd_run* r, double dt) {
+    if (!r) return fail(MHD_ERR_INVALID, "null run");
+    if (int rc = step_grids(r, r->U[0], r->U[1], r->U[2], dt)) return rc;
+    std::swap(r->U[0], r->U[2]);
+    r->acc_valid = true;
+    return MHD_OK;
+}
+
+int mhd_step_to(mhd_run* r, double t_end, int max_steps, double* t_out, int* steps_out, double* dts, double* chs) {
+    if (!r) return fail(MHD_ERR_INVALID, "null run");
+    double t = 0.0;
+    int steps = 0, rc = MHD_OK;
+    while (t < t_end && steps < max_steps) {  // main_cpu.cpp:196
+        double dt_raw = 0.0;
+        if ((rc = mhd_compute_dt(r, &dt_raw)) != MHD_OK) break;
+        const double dt = std::min(dt_raw, t_end - t);
+        if (!std::isfinite(dt) || dt <= 0.0) { rc = fail(MHD_ERR_DT, "dt non-finite or <= 0"); break; }
+        if (dt < 1.0e-8 * t_end) { rc = fail(MHD_ERR_DT_FLOOR, "dt below 1e-8 t_end: numerical blow-up"); break; }
+        if ((rc = mhd_advance(r, dt)) != MHD_OK) break;
+        if (dts) dts[steps] = dt;
+        if (chs) chs[steps] = r->ch;
+        t += dt;
+        ++steps;
+    }
+    if (t_out) *t_out = t;
+    if (steps_out) *steps_out = steps;
+    return rc;
+}
+
+int mhd_apply_boundary(mhd_run* r) {
+    if (!r) return fail(MHD_ERR_INVALID, "null run");
+    // In place: Dirichlet sides read the grid's own (unchanged) ghosts.
+    MHD_HIP(mhd::launch_bc(r->U[0], r->U[0], r->d.nx, r->d.ny, (long long)r->cells, r->d.bc, r->stream()));
+    return MHD_OK;
+}
+
+int mhd_compute_dt_grid(mhd_run* r, const double* U, double* dt) {
+    if (!r || !U || !dt) return fail(MHD_ERR_INVALID, "null argument");
+    return dt_from_state(r, U, false, dt);
+}
+
+int mhd_advance_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt) {
+    if (!r || !Uold || !Utmp || !Unew) return fail(MHD_ERR_INVALID, "null argument");
+    if (Uold == Unew || Uold == Utmp || Utmp == Unew) return fail(MHD_ERR_INVALID, "grids must be distinct");
+    r->acc_valid = false;
+    return step_grids(r, Uold, Utmp, Unew, dt);
+}
+
+int mhd_profile(mhd_run* r, int enable) {
+    if (!r) return fail(MHD_ERR_INVALID, "null run");
+    MHD_HIP(hipStreamSynchronize(r->stream()));
+    for (hipEvent_t ev : r->events) hipEventDestroy(ev);
+    r->events.clear();
+    r->prof_steps = 0;
+    r->profiling = enable != 0;
+    return MHD_OK;
+}
+
+int mhd_profile_read(mhd_run* r, double* ms4, int* steps) {
+    if (!r || !ms4) return fail(MHD_ERR_INVALID, "null argument");
+    MHD_HIP(hipStreamSynchronize(r->stream()));
+    for (int k = 0; k < 4; ++k) ms4[k] = 0.0;
+    for (size_t s = 0; s + 5 <= r->events.size(); s += 5)
+        for (int k = 0; k < 4; ++k) {
+            float ms = 0.0f;
+            MHD_HIP(hipEventElapsedTime(&ms, r->events[s + k], r->events[s + k + 1]));
+            ms4[k] += ms;
+        }
+    if (steps) *steps = r->prof_steps;
+    return MHD_OK;
+}
+
+int mhd_sync(mhd_run* r) {
+    if (!r) return fail(MHD_ERR_INVALID, "null run");
+    MHD_HIP(hipStreamSynchronize(r->stream()));
+    return MHD_OK;
+}
+
+}  // extern "C"

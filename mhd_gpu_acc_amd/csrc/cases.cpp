@@ -1,0 +1,186 @@
+// Problem set-up of the drop-in interface: the case table (src/test_cases.cpp:178-286)
+// and the analytic initial states (src/test_cases.cpp:24-148, src/init.cpp:34-43),
+// plus the cases this build adds (unit-box Orszag-Tang, MHD shock-cloud,
+// transmissive Brio-Wu, the batch Orszag-Tang family and the Gaussian bump).
+//
+// The initial states are evaluated on the host with the C library's sin, tanh,
+// exp and sqrt, as the reference does, so a state built here is bitwise the
+// state the reference (or any CPU code calling the same libm) builds.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#include "../../include/mhd_hip.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;  // M_PI (test_cases.cpp:15-17)
+
+struct State {
+    double q[9];
+};
+
+// make_mhd (test_cases.cpp:24-35): conserved state from primitives, psi = 0.
+State from_primitives(double rho, double u, double v, double w, double bx, double by, double bz, double p,
+                      double gamma) {
+    const double b2 = bx * bx + by * by + bz * bz;
+    const double ke = 0.5 * rho * (u * u + v * v + w * w);
+    const double e = p / (gamma - 1.0) + ke + 0.5 * b2;
+    return State{{rho, rho * u, rho * v, rho * w, bx, by, bz, e, 0.0}};
+}
+
+// Counter-based SplitMix64 (Steele, Lea & Flood 2014): one mixing step of the
+// state seed ^ (j << 32 | i); the top 53 bits give a uniform double in [0, 1).
+uint64_t splitmix_next(uint64_t& s) {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double uniform_at(uint64_t seed, int i, int j) {
+    uint64_t s = seed ^ ((static_cast<uint64_t>(static_cast<uint32_t>(j)) << 32) |
+                         static_cast<uint64_t>(static_cast<uint32_t>(i)));
+    return static_cast<double>(splitmix_next(s) >> 11) * 0x1.0p-53;
+}
+
+// Kelvin-Helmholtz (test_cases.cpp:43-63)
+State kelvin_helmholtz(double x, double y) {
+    const double ux = 5.0 * (std::tanh(20.0 * (y + 0.5)) - (std::tanh(20.0 * (y - 0.5)) + 1.0));
+    const double uy = 0.25 * std::sin(2.0 * kPi * x) *
+                      (std::exp(-100.0 * (y + 0.5) * (y + 0.5)) - std::exp(-100.0 * (y - 0.5) * (y - 0.5)));
+    return from_primitives(1.0, ux, uy, 0.0, 1.0, 0.0, 0.0, 50.0, 1.4);
+}
+
+// Mach 1.22 shock-bubble, air/helium (test_cases.cpp:76-106)
+State shock_bubble(double x, double y) {
+    const double gamma = 1.4, rho_air = 1.29, rho_he = 0.214, p0 = 1.01325e5, ms = 1.22;
+    const double c1 = std::sqrt(gamma * p0 / rho_air);
+    const double vs = ms * c1;
+    const double rho2 = rho_air * (gamma + 1.0) * ms * ms / ((gamma - 1.0) * ms * ms + 2.0);
+    const double p2 = p0 * (2.0 * gamma * ms * ms - (gamma - 1.0)) / (gamma + 1.0);
+    const double u2 = vs * (1.0 - rho_air / rho2);
+    if (x < 0.005) return from_primitives(rho2, u2, 0.0, 0.0, 0.0, 0.0, 0.0, p2, gamma);
+    const double r2 = (x - 0.035) * (x - 0.035) + (y - 0.0445) * (y - 0.0445);
+    if (r2 < 0.025 * 0.025) return from_primitives(rho_he, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, p0, gamma);
+    return from_primitives(rho_air, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, p0, gamma);
+}
+
+// Brio-Wu (test_cases.cpp:117-125)
+State brio_wu(double x) {
+    if (x < 0.5) return from_primitives(1.0, 0.0, 0.0, 0.0, 0.75, 1.0, 0.0, 1.0, 2.0);
+    return from_primitives(0.125, 0.0, 0.0, 0.0, 0.75, -1.0, 0.0, 0.1, 2.0);
+}
+
+// Orszag-Tang on [0, 2 pi]^2 (test_cases.cpp:139-148)
+State orszag_tang(double x, double y) {
+    const double gamma = 5.0 / 3.0;
+    return from_primitives(gamma * gamma, -std::sin(y), std::sin(x), 0.0, -std::sin(y), std::sin(2.0 * x), 0.0,
+                           gamma, gamma);
+}
+
+// Orszag-Tang on [0,1]^2: rho = 25/(36 pi), p = 5/(12 pi), u = -sin 2 pi y,
+// v = sin 2 pi x, B = b_scale (-sin 2 pi y, sin 4 pi x) / sqrt(4 pi).  With
+// `ripple`, rho *= 1 + pert_amp * sum_m a_m sin(2 pi (kx x + ky y) + phi_m) / sum |a_m|;
+// with `bump`, rho += bump_amp exp(-r^2 / (2 sigma^2)) (p held, E recomputed).
+State orszag_tang_unit(double x, double y, double gamma, const mhd_ic_params* p, bool ripple, bool bump) {
+    double rho = 25.0 / (36.0 * kPi);
+    const double pr = 5.0 / (12.0 * kPi);
+    const double b0 = (p ? p->b_scale : 1.0) / std::sqrt(4.0 * kPi);
+    if (ripple && p->n_modes > 0) {
+        double s = 0.0, norm = 0.0;
+        for (int m = 0; m < p->n_modes; ++m) {
+            s += p->mode_amp[m] * std::sin(2.0 * kPi * (p->mode_kx[m] * x + p->mode_ky[m] * y) + p->mode_phase[m]);
+            norm += std::fabs(p->mode_amp[m]);
+        }
+        rho = rho * (1.0 + p->pert_amp * (s / norm));
+    }
+    if (bump) {
+        const double r2 = (x - p->bump_cx) * (x - p->bump_cx) + (y - p->bump_cy) * (y - p->bump_cy);
+        rho = rho + p->bump_amp * std::exp(-r2 / (2.0 * p->bump_sigma * p->bump_sigma));
+    }
+    const double sy = std::sin(2.0 * kPi * y);
+    const double sx = std::sin(2.0 * kPi * x);
+    return from_primitives(rho, -sy, sx, 0.0, -sy * b0, std::sin(4.0 * kPi * x) * b0, 0.0, pr, gamma);
+}
+
+// MHD shock-cloud on [0,2]x[0,1]: post-shock slab x < 0.6 moving right into an
+// ambient state; a dense bubble r < 0.15 at (0.8, 0.5) in pressure balance with
+// +-1 % seeded density noise.
+State shock_cloud(double x, double y, int i, int j, uint64_t seed) {
+    const double gamma = 5.0 / 3.0;
+    if (x < 0.6) return from_primitives(3.86859, 11.2536, 0.0, 0.0, 0.0, 2.1826, -2.1826, 167.345, gamma);
+    const double r2 = (x - 0.8) * (x - 0.8) + (y - 0.5) * (y - 0.5);
+    if (r2 < 0.15 * 0.15) {
+        const double rho = 10.0 * (1.0 + 0.01 * (2.0 * uniform_at(seed, i, j) - 1.0));
+        return from_primitives(rho, 0.0, 0.0, 0.0, 0.0, 0.56418, 0.56418, 1.0, gamma);
+    }
+    return from_primitives(1.0, 0.0, 0.0, 0.0, 0.0, 0.56418, 0.56418, 1.0, gamma);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhd_case_config_get(int case_id, int n, mhd_case_config* c) {
+    if (!c || n < 1) return MHD_ERR_INVALID;
+    const double two_pi = 2.0 * kPi;
+    auto set = [&](int nx, int ny, double x0, double x1, double y0, double y1, double cfl, double t_end, double g,
+                   int l, int r, int b, int t) {
+        c->nx = nx * n; c->ny = ny * n; c->ng = 2;
+        c->x_min = x0; c->x_max = x1; c->y_min = y0; c->y_max = y1;
+        c->cfl = cfl; c->t_end = t_end; c->gamma = g;
+        c->bc[0] = l; c->bc[1] = r; c->bc[2] = b; c->bc[3] = t;
+    };
+    switch (case_id) {
+        case MHD_CASE_KELVIN_HELMHOLTZ: set(200, 400, 0.0, 1.0, -1.0, 1.0, 0.3, 0.5, 1.4, 0, 0, 0, 0); break;
+        case MHD_CASE_SHOCK_BUBBLE: {
+            // t_end = last snapshot of the Haas-Sturtevant schedule (test_cases.cpp:199-225)
+            const double vs = 1.22 * std::sqrt(1.4 * 1.01325e5 / 1.29);
+            const double t0 = (0.035 - 0.025 - 0.005) / vs, tau = 0.025 / vs;
+            set(500, 197, 0.0, 0.225, 0.0, 0.089, 0.4, t0 + 19.0 * tau, 1.4, 1, 1, 1, 1);
+            break;
+        }
+        case MHD_CASE_BRIO_WU: set(800, 4, 0.0, 1.0, 0.0, 4.0 / 800.0, 0.4, 0.1, 2.0, 1, 1, 0, 0); break;
+        case MHD_CASE_ORSZAG_TANG: set(192, 192, 0.0, two_pi, 0.0, two_pi, 0.4, two_pi, 5.0 / 3.0, 0, 0, 0, 0); break;
+        case MHD_CASE_ORSZAG_TANG_UNIT:
+        case MHD_CASE_OT_FAMILY:
+        case MHD_CASE_OT_BUMP: set(1024, 1024, 0.0, 1.0, 0.0, 1.0, 0.8, 0.5, 5.0 / 3.0, 0, 0, 0, 0); break;
+        case MHD_CASE_SHOCK_CLOUD: set(2048, 1024, 0.0, 2.0, 0.0, 1.0, 0.8, 1.0, 5.0 / 3.0, 1, 1, 1, 1); break;
+        case MHD_CASE_BRIO_WU_T: set(800, 4, 0.0, 1.0, 0.0, 4.0 / 800.0, 0.8, 0.1, 2.0, 1, 1, 1, 1); break;
+        default: return MHD_ERR_INVALID;
+    }
+    return MHD_OK;
+}
+
+int mhd_init_state_host(int case_id, const mhd_case_config* gm, const mhd_ic_params* p, double* aos) {
+    if (!gm || !aos || gm->nx < 1 || gm->ny < 1 || gm->ng != 2) return MHD_ERR_INVALID;
+    if ((case_id == MHD_CASE_OT_FAMILY || case_id == MHD_CASE_OT_BUMP) && !p) return MHD_ERR_INVALID;
+    const int tnx = gm->nx + 2 * gm->ng, tny = gm->ny + 2 * gm->ng;
+    const double dx = (gm->x_max - gm->x_min) / static_cast<double>(gm->nx);
+    const double dy = (gm->y_max - gm->y_min) / static_cast<double>(gm->ny);
+    const double gamma_family = p ? p->gamma : 5.0 / 3.0;
+    for (int j = 0; j < tny; ++j) {
+        const double y = gm->y_min + (static_cast<double>(j - gm->ng) + 0.5) * dy;  // init.cpp:13-15
+        for (int i = 0; i < tnx; ++i) {
+            const double x = gm->x_min + (static_cast<double>(i - gm->ng) + 0.5) * dx;
+            State s;
+            switch (case_id) {
+                case MHD_CASE_KELVIN_HELMHOLTZ: s = kelvin_helmholtz(x, y); break;
+                case MHD_CASE_SHOCK_BUBBLE: s = shock_bubble(x, y); break;
+                case MHD_CASE_BRIO_WU:
+                case MHD_CASE_BRIO_WU_T: s = brio_wu(x); break;
+                case MHD_CASE_ORSZAG_TANG: s = orszag_tang(x, y); break;
+                case MHD_CASE_ORSZAG_TANG_UNIT: s = orszag_tang_unit(x, y, 5.0 / 3.0, nullptr, false, false); break;
+                case MHD_CASE_SHOCK_CLOUD: s = shock_cloud(x, y, i, j, p ? p->seed : 0); break;
+                case MHD_CASE_OT_FAMILY: s = orszag_tang_unit(x, y, gamma_family, p, true, false); break;
+                case MHD_CASE_OT_BUMP: s = orszag_tang_unit(x, y, 5.0 / 3.0, p, false, true); break;
+                default: return MHD_ERR_INVALID;
+            }
+            std::memcpy(aos + 9 * (static_cast<size_t>(j) * tnx + i), s.q, sizeof s.q);
+        }
+    }
+    return MHD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,31 @@
+// Host-side launch interface of the step kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mhd {
+
+struct StepArgs {
+    const double* Uold;
+    double* Utmp;
+    double* Unew;
+    int nx, ny;
+    long long ps;            // plane stride in doubles
+    double dx, dy, dt;
+    double gamma, ch;
+    double damp;             // exp(-dt ch / cr), computed on the host with libm
+    int solver, limiter;
+    int bc[4];
+    int seg;                 // y-sweep rows per lane segment (0 = default)
+    double* frame;           // undamped psi of the interior frame (2*2*(nx+ny) doubles)
+    unsigned long long* acc; // max signal speed accumulator (bit pattern of a double >= 0)
+    hipStream_t stream;
+    hipEvent_t* ev;          // null, or 5 events recorded around the 4 launches
+};
+
+hipError_t launch_step(const StepArgs& a);
+hipError_t launch_cfl(const double* U, int nx, int ny, long long ps, double gamma, double ch,
+                      unsigned long long* acc, hipStream_t s);
+hipError_t launch_bc(double* U, const double* prev, int nx, int ny, long long ps, const int* bc, hipStream_t s);
+
+}  // namespace mhd

@@ -1,0 +1,422 @@
+// Point physics of the 2-D Dedner GLM-MHD system for the gfx950 kernels.
+//
+// States are held in a rotated frame: for a sweep along x the "normal"
+// components are (rho u, Bx) and the tangential ones (rho v, By); along y
+// they are exchanged.  Every expression of the reference (head/physics.hpp,
+// head/riemann.hpp, src/cpu/solver_cpu.cpp) is invariant under that exchange
+// up to commutation of a single + or *, which IEEE arithmetic preserves
+// exactly, so one code path serves both sweeps bit-for-bit.
+//
+// Evaluation order follows the reference literally (C++ left-to-right
+// association; `v/s` on a 9-vector is `(1/s)*v`, types.hpp:104-106;
+// std::min/std::max NaN rules).  Built with -ffp-contract=off: no fused
+// multiply-adds are formed, so every product and sum rounds as on the CPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#pragma clang fp contract(off)
+
+namespace mhd {
+
+#define MHD_DI __device__ __forceinline__
+
+// Conserved state, rotated: (rho, m_n, m_t, m_z, B_n, B_t, B_z, E, psi).
+struct Cons {
+    double r, mn, mt, mz, bn, bt, bz, e, psi;
+};
+// Primitive state, rotated: (rho, u_n, u_t, u_z, B_n, B_t, B_z, p, psi).
+struct Prim {
+    double r, un, ut, uz, bn, bt, bz, p, psi;
+};
+
+// Per-launch physics constants.
+struct Phys {
+    double gamma;   // adiabatic index
+    double gm1;     // gamma - 1 (physics.hpp:258, 268)
+    double ch;      // GLM cleaning speed of this step
+    double ch2;     // ch * ch (physics.hpp:307)
+    double half_ch; // 0.5 * ch (riemann.hpp:204, 441, 558)
+    double half_over_ch;  // 0.5 / ch (riemann.hpp:557)
+};
+
+MHD_DI bool finite(double x) { return __builtin_isfinite(x); }
+MHD_DI double std_min(double a, double b) { return (b < a) ? b : a; }  // std::min
+MHD_DI double std_max(double a, double b) { return (a < b) ? b : a; }  // std::max
+
+// physics.hpp:69-78
+MHD_DI Prim to_prim(const Cons& u, const Phys& ph) {
+    const double inv = 1.0 / u.r;
+    Prim w;
+    w.r = u.r;
+    w.un = u.mn * inv;
+    w.ut = u.mt * inv;
+    w.uz = u.mz * inv;
+    w.bn = u.bn;
+    w.bt = u.bt;
+    w.bz = u.bz;
+    const double b2 = u.bn * u.bn + u.bt * u.bt + u.bz * u.bz;
+    const double ke = 0.5 * u.r * (w.un * w.un + w.ut * w.ut + w.uz * w.uz);
+    w.p = ph.gm1 * (u.e - ke - 0.5 * b2);
+    w.psi = u.psi;
+    return w;
+}
+
+// physics.hpp:83-89
+MHD_DI Cons to_cons(const Prim& w, const Phys& ph) {
+    const double b2 = w.bn * w.bn + w.bt * w.bt + w.bz * w.bz;
+    const double ke = 0.5 * w.r * (w.un * w.un + w.ut * w.ut + w.uz * w.uz);
+    Cons u;
+    u.r = w.r;
+    u.mn = w.r * w.un;
+    u.mt = w.r * w.ut;
+    u.mz = w.r * w.uz;
+    u.bn = w.bn;
+    u.bt = w.bt;
+    u.bz = w.bz;
+    u.e = w.p / ph.gm1 + ke + 0.5 * b2;
+    u.psi = w.psi;
+    return u;
+}
+
+// physics.hpp:105-164.  `w` must be to_prim(u): the reference recomputes the same
+// expressions inside flux_x/flux_y, so passing them in is bitwise neutral.
+MHD_DI Cons flux(const Cons& u, const Prim& w, const Phys& ph) {
+    const double b2 = u.bn * u.bn + u.bt * u.bt + u.bz * u.bz;
+    const double pt = w.p + 0.5 * b2;
+    const double bu = u.bn * w.un + u.bt * w.ut + u.bz * w.uz;
+    Cons f;
+    f.r = u.mn;
+    f.mn = u.mn * w.un + pt - u.bn * u.bn;
+    f.mt = u.mn * w.ut - u.bn * u.bt;
+    f.mz = u.mn * w.uz - u.bn * u.bz;
+    f.bn = u.psi;
+    f.bt = u.bt * w.un - u.bn * w.ut;
+    f.bz = u.bz * w.un - u.bn * w.uz;
+    f.e = (u.e + pt) * w.un - u.bn * bu;
+    f.psi = ph.ch2 * u.bn;
+    return f;
+}
+
+// physics.hpp:175-190 (fast magnetosonic speed along the normal)
+MHD_DI double fast_speed(const Prim& w, const Phys& ph) {
+    const double a2 = ph.gamma * w.p / w.r;
+    const double b2 = (w.bn * w.bn + w.bt * w.bt + w.bz * w.bz) / w.r;
+    const double bn2 = w.bn * w.bn / w.r;
+    const double disc = (a2 + b2) * (a2 + b2) - 4.0 * a2 * bn2;
+    return sqrt(0.5 * (a2 + b2 + sqrt(disc > 0.0 ? disc : 0.0)));
+}
+
+// riemann.hpp:54-57
+MHD_DI bool prim_physical(const Prim& w) {
+    return finite(w.r) && finite(w.p) && w.r > 0.0 && w.p > 0.0;
+}
+
+// solver_cpu.cpp:49-56 (floors 1e-12, every component finite)
+MHD_DI bool is_physical(const Prim& w) {
+    return finite(w.r) && w.r > 1.0e-12 && finite(w.p) && w.p > 1.0e-12 && finite(w.un) && finite(w.ut) &&
+           finite(w.uz) && finite(w.bn) && finite(w.bt) && finite(w.bz) && finite(w.psi);
+}
+
+MHD_DI Cons half_sum(const Cons& a, const Cons& b) {
+    return Cons{0.5 * (a.r + b.r), 0.5 * (a.mn + b.mn), 0.5 * (a.mt + b.mt), 0.5 * (a.mz + b.mz),
+                0.5 * (a.bn + b.bn), 0.5 * (a.bt + b.bt), 0.5 * (a.bz + b.bz), 0.5 * (a.e + b.e),
+                0.5 * (a.psi + b.psi)};
+}
+
+// Star state from rotated components (riemann.hpp:219-230, 417-437).
+MHD_DI Cons star(double rs, double uns, double uts, double uws, double bns, double bts, double bws, double es,
+                 double psis) {
+    return Cons{rs, rs * uns, rs * uts, rs * uws, bns, bts, bws, es, psis};
+}
+
+// f + s * (a - b), componentwise
+MHD_DI Cons fan(const Cons& f, double s, const Cons& a, const Cons& b) {
+    return Cons{f.r + s * (a.r - b.r),    f.mn + s * (a.mn - b.mn), f.mt + s * (a.mt - b.mt),
+                f.mz + s * (a.mz - b.mz), f.bn + s * (a.bn - b.bn), f.bt + s * (a.bt - b.bt),
+                f.bz + s * (a.bz - b.bz), f.e + s * (a.e - b.e),    f.psi + s * (a.psi - b.psi)};
+}
+
+// ---------------------------------------------------------------------------------
+// HLL (riemann.hpp:70-107).  FL/FR and the prims are passed in.
+// ---------------------------------------------------------------------------------
+MHD_DI Cons hll(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur, const Prim& vr, const Cons& fr,
+                const Phys& ph) {
+    if (!prim_physical(vl) || !prim_physical(vr)) return half_sum(fl, fr);
+    const double cfl = fast_speed(vl, ph);
+    const double cfr = fast_speed(vr, ph);
+    const double sl = fmin(fmin(vl.un - cfl, vr.un - cfr), -ph.ch);
+    const double sr = fmax(fmax(vl.un + cfl, vr.un + cfr), ph.ch);
+    if (sl >= 0.0) return fl;
+    if (sr <= 0.0) return fr;
+    const double den = sr - sl;
+    if (fabs(den) < 1.0e-14) return half_sum(fl, fr);
+    const double inv = 1.0 / den;
+    const double ss = sl * sr;
+    return Cons{inv * (sr * fl.r - sl * fr.r + ss * (ur.r - ul.r)),
+                inv * (sr * fl.mn - sl * fr.mn + ss * (ur.mn - ul.mn)),
+                inv * (sr * fl.mt - sl * fr.mt + ss * (ur.mt - ul.mt)),
+                inv * (sr * fl.mz - sl * fr.mz + ss * (ur.mz - ul.mz)),
+                inv * (sr * fl.bn - sl * fr.bn + ss * (ur.bn - ul.bn)),
+                inv * (sr * fl.bt - sl * fr.bt + ss * (ur.bt - ul.bt)),
+                inv * (sr * fl.bz - sl * fr.bz + ss * (ur.bz - ul.bz)),
+                inv * (sr * fl.e - sl * fr.e + ss * (ur.e - ul.e)),
+                inv * (sr * fl.psi - sl * fr.psi + ss * (ur.psi - ul.psi))};
+}
+
+// ---------------------------------------------------------------------------------
+// HLLC (riemann.hpp:122-241)
+// ---------------------------------------------------------------------------------
+MHD_DI Cons hllc(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur, const Prim& vr, const Cons& fr,
+                 const Phys& ph) {
+    if (!prim_physical(vl) || !prim_physical(vr)) return hll(ul, vl, fl, ur, vr, fr, ph);
+    const double cfl = fast_speed(vl, ph);
+    const double cfr = fast_speed(vr, ph);
+    const double sl = fmin(vl.un - cfl, vr.un - cfr);
+    const double sr = fmax(vl.un + cfl, vr.un + cfr);
+    if (sl >= 0.0) return fl;
+    if (sr <= 0.0) return fr;
+    const double bl2 = vl.bn * vl.bn + vl.bt * vl.bt + vl.bz * vl.bz;
+    const double br2 = vr.bn * vr.bn + vr.bt * vr.bt + vr.bz * vr.bz;
+    const double ptl = vl.p + 0.5 * bl2;
+    const double ptr = vr.p + 0.5 * br2;
+    const double num = (sr - vr.un) * vr.r * vr.un - (sl - vl.un) * vl.r * vl.un + ptl - ptr;
+    const double den = (sr - vr.un) * vr.r - (sl - vl.un) * vl.r;
+    if (fabs(den) < 1.0e-14) return hll(ul, vl, fl, ur, vr, fr, ph);
+    const double sm = num / den;
+    const double rls = vl.r * (sl - vl.un) / (sl - sm);
+    const double rrs = vr.r * (sr - vr.un) / (sr - sm);
+    if (rls <= 0.0 || rrs <= 0.0 || !finite(rls) || !finite(rrs)) return hll(ul, vl, fl, ur, vr, fr, ph);
+    const double psim = 0.5 * (vl.psi + vr.psi) - ph.half_ch * (vr.bn - vl.bn);
+    if (sm >= 0.0) {
+        const double ptls = ptl + vl.r * (sl - vl.un) * (sm - vl.un);
+        const double bu = vl.bn * vl.un + vl.bt * vl.ut + vl.bz * vl.uz;
+        const double bus = vl.bn * sm + vl.bt * vl.ut + vl.bz * vl.uz;
+        const double es = ((sl - vl.un) * ul.e - ptl * vl.un + ptls * sm + vl.bn * (bus - bu)) / (sl - sm);
+        return fan(fl, sl, star(rls, sm, vl.ut, vl.uz, vl.bn, vl.bt, vl.bz, es, psim), ul);
+    }
+    const double ptrs = ptr + vr.r * (sr - vr.un) * (sm - vr.un);
+    const double bu = vr.bn * vr.un + vr.bt * vr.ut + vr.bz * vr.uz;
+    const double bus = vr.bn * sm + vr.bt * vr.ut + vr.bz * vr.uz;
+    const double es = ((sr - vr.un) * ur.e - ptr * vr.un + ptrs * sm + vr.bn * (bus - bu)) / (sr - sm);
+    return fan(fr, sr, star(rrs, sm, vr.ut, vr.uz, vr.bn, vr.bt, vr.bz, es, psim), ur);
+}
+
+// ---------------------------------------------------------------------------------
+// HLLD (riemann.hpp:253-480), Miyoshi-Kusano with B_n allowed to jump (GLM).
+// ---------------------------------------------------------------------------------
+struct SideStar {
+    double ut, uw, bt, bw, e;  // single-star tangential velocity/field and energy
+    double bu;                 // B.u of the single-star state
+};
+
+// L* / R* states (riemann.hpp:341-373).  s = S_L or S_R.
+MHD_DI SideStar side_star(const Cons& u, const Prim& v, double s, double sm, double ptl, double ptstar) {
+    SideStar o;
+    const double d = v.r * (s - v.un) * (s - sm) - v.bn * v.bn;
+    if (fabs(d) < 1.0e-14) {
+        o.ut = v.ut; o.uw = v.uz; o.bt = v.bt; o.bw = v.bz;
+    } else {
+        o.ut = v.ut - v.bn * v.bt * (sm - v.un) / d;
+        o.uw = v.uz - v.bn * v.bz * (sm - v.un) / d;
+        const double k = v.r * (s - v.un) * (s - v.un) - v.bn * v.bn;
+        o.bt = v.bt * k / d;
+        o.bw = v.bz * k / d;
+    }
+    const double bu = v.bn * v.un + v.bt * v.ut + v.bz * v.uz;
+    o.bu = v.bn * sm + o.bt * o.ut + o.bw * o.uw;
+    o.e = (u.e * (s - v.un) - ptl * v.un + ptstar * sm + v.bn * (o.bu - bu)) / (s - sm);
+    return o;
+}
+
+MHD_DI Cons hlld(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur, const Prim& vr, const Cons& fr,
+                 const Phys& ph) {
+    if (!prim_physical(vl) || !prim_physical(vr)) return hll(ul, vl, fl, ur, vr, fr, ph);
+    const double cfl = fast_speed(vl, ph);
+    const double cfr = fast_speed(vr, ph);
+    const double sl = fmin(vl.un - cfl, vr.un - cfr);
+    const double sr = fmax(vl.un + cfl, vr.un + cfr);
+    if (fabs(sr - sl) < 1.0e-14) return hll(ul, vl, fl, ur, vr, fr, ph);
+    const double bl2 = vl.bn * vl.bn + vl.bt * vl.bt + vl.bz * vl.bz;
+    const double br2 = vr.bn * vr.bn + vr.bt * vr.bt + vr.bz * vr.bz;
+    const double ptl = vl.p + 0.5 * bl2;
+    const double ptr = vr.p + 0.5 * br2;
+    const double num = (sr - vr.un) * vr.r * vr.un - (sl - vl.un) * vl.r * vl.un + ptl - ptr;
+    const double den = (sr - vr.un) * vr.r - (sl - vl.un) * vl.r;
+    if (fabs(den) < 1.0e-14) return hll(ul, vl, fl, ur, vr, fr, ph);
+    const double sm = num / den;
+    const double rls = vl.r * (sl - vl.un) / (sl - sm);
+    const double rrs = vr.r * (sr - vr.un) / (sr - sm);
+    if (rls <= 0.0 || rrs <= 0.0 || !finite(rls) || !finite(rrs)) return hll(ul, vl, fl, ur, vr, fr, ph);
+    // The outermost regions are decided before any star state is needed
+    // (riemann.hpp:448-449); the values they return do not depend on it.
+    if (sl >= 0.0) return fl;
+    if (sr <= 0.0) return fr;
+    const double ptstar = ptl + vl.r * (sl - vl.un) * (sm - vl.un);
+    const double sql = sqrt(rls);
+    const double sqr = sqrt(rrs);
+    const double sls = sm - fabs(vl.bn) / sql;
+    const double srs = sm + fabs(vr.bn) / sqr;
+    const double psim = 0.5 * (vl.psi + vr.psi) - ph.half_ch * (vr.bn - vl.bn);
+    const SideStar L = side_star(ul, vl, sl, sm, ptl, ptstar);
+    const SideStar R = side_star(ur, vr, sr, sm, ptr, ptstar);
+    const bool left = sm >= 0.0;
+    const bool outer = left ? (sls >= 0.0) : (srs <= 0.0);
+    const Cons& us = left ? ul : ur;
+    const Cons& fs = left ? fl : fr;
+    const SideStar& S = left ? L : R;
+    const double s = left ? sl : sr;
+    const double rs = left ? rls : rrs;
+    const double bn = left ? vl.bn : vr.bn;
+    const Cons u1 = star(rs, sm, S.ut, S.uw, bn, S.bt, S.bw, S.e, psim);
+    if (outer) return fan(fs, s, u1, us);  // L* or R* region
+    // Double-star states (riemann.hpp:376-413); sign of B_n from the left state.
+    const double sgn = (vl.bn >= 0.0) ? 1.0 : -1.0;
+    double ut2, uw2, bt2, bw2, e2;
+    if (fabs(sql + sqr) < 1.0e-14) {
+        ut2 = S.ut; uw2 = S.uw; bt2 = S.bt; bw2 = S.bw; e2 = S.e;
+    } else {
+        const double inv = 1.0 / (sql + sqr);
+        ut2 = (sql * L.ut + sqr * R.ut + sgn * (R.bt - L.bt)) * inv;
+        uw2 = (sql * L.uw + sqr * R.uw + sgn * (R.bw - L.bw)) * inv;
+        bt2 = (sql * R.bt + sqr * L.bt + sgn * sql * sqr * (R.ut - L.ut)) * inv;
+        bw2 = (sql * R.bw + sqr * L.bw + sgn * sql * sqr * (R.uw - L.uw)) * inv;
+        const double bu2 = bn * sm + bt2 * ut2 + bw2 * uw2;
+        e2 = left ? (S.e - sql * sgn * (bu2 - S.bu)) : (S.e + sqr * sgn * (bu2 - S.bu));
+    }
+    const double s2 = left ? sls : srs;
+    const Cons u2 = star(rs, sm, ut2, uw2, bn, bt2, bw2, e2, psim);
+    const Cons f1 = fan(fs, s, u1, us);
+    return fan(f1, s2, u2, u1);  // L** or R** region
+}
+
+// ---------------------------------------------------------------------------------
+// FORCE (riemann.hpp:486-525)
+// ---------------------------------------------------------------------------------
+MHD_DI Cons force(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur, const Prim& vr, const Cons& fr,
+                  const Phys& ph) {
+    if (!prim_physical(vl) || !prim_physical(vr)) return half_sum(fl, fr);
+    const double cfl = fast_speed(vl, ph);
+    const double cfr = fast_speed(vr, ph);
+    const double a = fmax(fmax(fabs(vl.un) + cfl, fabs(vr.un) + cfr), ph.ch);
+    if (a < 1.0e-14) return half_sum(fl, fr);
+    const double ha = 0.5 * a;
+    const double hia = 0.5 * (1.0 / a);
+    const Cons flf{0.5 * (fl.r + fr.r) - ha * (ur.r - ul.r),       0.5 * (fl.mn + fr.mn) - ha * (ur.mn - ul.mn),
+                   0.5 * (fl.mt + fr.mt) - ha * (ur.mt - ul.mt),   0.5 * (fl.mz + fr.mz) - ha * (ur.mz - ul.mz),
+                   0.5 * (fl.bn + fr.bn) - ha * (ur.bn - ul.bn),   0.5 * (fl.bt + fr.bt) - ha * (ur.bt - ul.bt),
+                   0.5 * (fl.bz + fr.bz) - ha * (ur.bz - ul.bz),   0.5 * (fl.e + fr.e) - ha * (ur.e - ul.e),
+                   0.5 * (fl.psi + fr.psi) - ha * (ur.psi - ul.psi)};
+    const Cons uri{0.5 * (ul.r + ur.r) - hia * (fr.r - fl.r),       0.5 * (ul.mn + ur.mn) - hia * (fr.mn - fl.mn),
+                   0.5 * (ul.mt + ur.mt) - hia * (fr.mt - fl.mt),   0.5 * (ul.mz + ur.mz) - hia * (fr.mz - fl.mz),
+                   0.5 * (ul.bn + ur.bn) - hia * (fr.bn - fl.bn),   0.5 * (ul.bt + ur.bt) - hia * (fr.bt - fl.bt),
+                   0.5 * (ul.bz + ur.bz) - hia * (fr.bz - fl.bz),   0.5 * (ul.e + ur.e) - hia * (fr.e - fl.e),
+                   0.5 * (ul.psi + ur.psi) - hia * (fr.psi - fl.psi)};
+    const Prim vri = to_prim(uri, ph);
+    if (!prim_physical(vri)) return flf;
+    return half_sum(flf, flux(uri, vri, ph));
+}
+
+// riemann.hpp:544-563: exact upwind flux of the (B_n, psi) subsystem.
+MHD_DI void glm_overwrite(Cons& f, const Cons& ul, const Cons& ur, const Phys& ph) {
+    if (ph.ch <= 0.0) return;
+    const double bstar = 0.5 * (ul.bn + ur.bn) - ph.half_over_ch * (ur.psi - ul.psi);
+    const double pstar = 0.5 * (ul.psi + ur.psi) - ph.half_ch * (ur.bn - ul.bn);
+    f.bn = pstar;
+    f.psi = ph.ch2 * bstar;
+}
+
+// riemann.hpp:568-592.  XDIR selects which rotated slot is the reference's
+// F.Bx in the finiteness test (:583-586): B_n on x sweeps, B_t on y sweeps.
+template <int SOLVER, bool XDIR>
+MHD_DI Cons riemann(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr, const Phys& ph) {
+    const Cons fl = flux(ul, vl, ph);
+    const Cons fr = flux(ur, vr, ph);
+    if (SOLVER == 0) return hll(ul, vl, fl, ur, vr, fr, ph);
+    Cons f = (SOLVER == 1) ? hllc(ul, vl, fl, ur, vr, fr, ph)
+           : (SOLVER == 2) ? hlld(ul, vl, fl, ur, vr, fr, ph)
+                           : force(ul, vl, fl, ur, vr, fr, ph);
+    const double fbx = XDIR ? f.bn : f.bt;
+    if (!finite(f.r) || !finite(f.e) || !finite(fbx) || !finite(f.psi)) return hll(ul, vl, fl, ur, vr, fr, ph);
+    glm_overwrite(f, ul, ur, ph);
+    return f;
+}
+
+// ---------------------------------------------------------------------------------
+// MUSCL-Hancock reconstruction (solver_cpu.cpp:76-119)
+// ---------------------------------------------------------------------------------
+// minmod (solver_cpu.cpp:26-29); van Leer 2ab/(a+b) on same-sign slopes.
+template <int LIMITER> MHD_DI double limit(double a, double b) {
+    if (a * b <= 0.0) return 0.0;
+    if (LIMITER == 0) return (a > 0.0) ? std_min(a, b) : std_max(a, b);
+    return 2.0 * a * b / (a + b);
+}
+
+// Interface states of one cell with their primitives (the Riemann solver needs
+// both, and the reference's guard already computes the primitive of the
+// candidate state, solver_cpu.cpp:65-70).
+struct Face {
+    Cons u;
+    Prim w;
+};
+
+template <int LIMITER>
+MHD_DI void reconstruct(const Prim& wm, const Prim& wc, const Prim& wp, double hdtd, const Phys& ph, Face& lo,
+                        Face& hi) {
+    Prim wl, wr;
+#define MHD_SLOPE(f)                                          \
+    {                                                         \
+        const double s = limit<LIMITER>(wc.f - wm.f, wp.f - wc.f); \
+        wl.f = wc.f - 0.5 * s;                                \
+        wr.f = wc.f + 0.5 * s;                                \
+    }
+    MHD_SLOPE(r) MHD_SLOPE(un) MHD_SLOPE(ut) MHD_SLOPE(uz) MHD_SLOPE(bn) MHD_SLOPE(bt) MHD_SLOPE(bz) MHD_SLOPE(p)
+    MHD_SLOPE(psi)
+#undef MHD_SLOPE
+    if (!is_physical(wl)) wl = wc;
+    if (!is_physical(wr)) wr = wc;
+    const Cons ul = to_cons(wl, ph);
+    const Cons ur = to_cons(wr, ph);
+    const Prim vl = to_prim(ul, ph);
+    const Prim vr = to_prim(ur, ph);
+    const Cons fl = flux(ul, vl, ph);
+    const Cons fr = flux(ur, vr, ph);
+    Cons cl, cr;
+#define MHD_HALF(f)                          \
+    {                                        \
+        const double h = hdtd * (fr.f - fl.f); \
+        cl.f = ul.f - h;                     \
+        cr.f = ur.f - h;                     \
+    }
+    MHD_HALF(r) MHD_HALF(mn) MHD_HALF(mt) MHD_HALF(mz) MHD_HALF(bn) MHD_HALF(bt) MHD_HALF(bz) MHD_HALF(e)
+    MHD_HALF(psi)
+#undef MHD_HALF
+    const Prim pcl = to_prim(cl, ph);
+    const Prim pcr = to_prim(cr, ph);
+    if (is_physical(pcl)) { lo.u = cl; lo.w = pcl; } else { lo.u = ul; lo.w = vl; }
+    if (is_physical(pcr)) { hi.u = cr; hi.w = pcr; } else { hi.u = ur; hi.w = vr; }
+}
+
+// Conservative update with the non-finite revert (solver_cpu.cpp:386-392).
+MHD_DI Cons update(const Cons& u, double dtd, const Cons& fp, const Cons& fm) {
+    Cons n{u.r - dtd * (fp.r - fm.r),    u.mn - dtd * (fp.mn - fm.mn), u.mt - dtd * (fp.mt - fm.mt),
+           u.mz - dtd * (fp.mz - fm.mz), u.bn - dtd * (fp.bn - fm.bn), u.bt - dtd * (fp.bt - fm.bt),
+           u.bz - dtd * (fp.bz - fm.bz), u.e - dtd * (fp.e - fm.e),    u.psi - dtd * (fp.psi - fm.psi)};
+    if (!finite(n.r) || !finite(n.e)) return u;
+    return n;
+}
+
+// Candidate of compute_dt for one cell (solver_cpu.cpp:291-300): the larger of the
+// x and y signal speeds, or 0 when the cell is skipped.  `u` is in the x frame.
+MHD_DI double cfl_candidate(const Cons& u, const Phys& ph) {
+    const Prim w = to_prim(u, ph);
+    if (!finite(w.r) || !finite(w.p) || w.r <= 0.0 || w.p <= 0.0) return 0.0;
+    const double sx = fmax(fabs(w.un) + fast_speed(w, ph), ph.ch);
+    Prim wy = w;
+    wy.un = w.ut; wy.ut = w.un; wy.bn = w.bt; wy.bt = w.bn;
+    const double sy = fmax(fabs(wy.un) + fast_speed(wy, ph), ph.ch);
+    if (finite(sx) && finite(sy)) return std_max(sx, sy);
+    return 0.0;
+}
+
+}  // namespace mhd

@@ -211,6 +211,29 @@ def run_loop(U: np.ndarray, case: Case, solver: int, limiter: int, max_steps: in
     return dict(U=U, dts=dts[:k].copy(), chs=chs[:k].copy(), nsteps=k, t=t.value, code=code, ch=ch.value)
 
 
+def run_loop_timed(U: np.ndarray, case: Case, solver: int, limiter: int, max_steps: int,
+                   t_end: float | None = None, cr: float = 0.18):
+    """run_loop with the wall time of the step loop only; returns (result dict, seconds)."""
+    U = np.ascontiguousarray(U.copy())
+    g = case.geom()
+    dts = np.zeros(max(max_steps, 1))
+    chs = np.zeros(max(max_steps, 1))
+    ch = C.c_double(0.0)
+    n = C.c_int(0)
+    t = C.c_double(0.0)
+    sec = C.c_double(0.0)
+    code = orc().orc_run_loop_timed(ptr(U), C.byref(g), C.c_double(case.cfl),
+                                    C.c_double(case.t_end if t_end is None else t_end), max_steps,
+                                    C.c_double(case.gamma), C.c_double(cr), solver, limiter, ibuf(case.bc),
+                                    C.byref(ch), ptr(dts), ptr(chs), C.byref(n), C.byref(t), C.byref(sec))
+    k = n.value
+    return dict(U=U, dts=dts[:k].copy(), chs=chs[:k].copy(), nsteps=k, t=t.value, code=code), sec.value
+
+
+def num_threads() -> int:
+    return int(orc().orc_num_threads())
+
+
 def tangent_run(U: np.ndarray, dU: np.ndarray, case: Case, dts, chs, solver: int, limiter: int,
                 bc=None, cr: float = 0.18):
     """Forward-mode derivative through len(dts) steps with frozen dt/c_h; returns (U, dU)."""

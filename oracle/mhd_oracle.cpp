@@ -9,10 +9,15 @@
 // tangent-linear model with branches frozen at the primal values.
 #include "mhd_oracle.hpp"
 
+#include <chrono>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
 #include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace {
 
@@ -837,9 +842,26 @@ void orc_copy_ghost_cells(const double* src, double* dst, const orc_geom* gm) {
 int orc_run_loop(double* U, const orc_geom* gm, double cfl, double t_end, int max_steps, double gamma, double cr,
                  int solver, int limiter, const int* bc, double* ch_inout, double* dts, double* chs, int* nsteps,
                  double* t_out) {
+    double s = 0.0;
+    return orc_run_loop_timed(U, gm, cfl, t_end, max_steps, gamma, cr, solver, limiter, bc, ch_inout, dts, chs,
+                              nsteps, t_out, &s);
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+int orc_run_loop_timed(double* U, const orc_geom* gm, double cfl, double t_end, int max_steps, double gamma,
+                       double cr, int solver, int limiter, const int* bc, double* ch_inout, double* dts,
+                       double* chs, int* nsteps, double* t_out, double* seconds) {
     const Geo g(*gm);
     Field<double> a = to_field(U, g), b = a, tmp = a;
     Workspace<double> ws;
+    const auto t0 = std::chrono::steady_clock::now();
     double t = 0.0, ch = *ch_inout;
     int step = 0, code = 0;
     while (t < t_end && step < max_steps) {
@@ -855,6 +877,7 @@ int orc_run_loop(double* U, const orc_geom* gm, double cfl, double t_end, int ma
         if (chs) chs[step] = ch;
         ++step;
     }
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     from_field(a, U);
     *ch_inout = ch;
     *nsteps = step;

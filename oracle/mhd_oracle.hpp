@@ -91,6 +91,13 @@ int orc_run_loop(double* U, const orc_geom* g, double cfl, double t_end, int max
                  double cr, int solver, int limiter, const int* bc, double* ch_inout, double* dts,
                  double* chs, int* nsteps, double* t_out);
 
+// orc_run_loop with the time of the step loop alone (steady clock, as
+// main_cpu.cpp:191,254-257 times it) returned in *seconds.
+int orc_run_loop_timed(double* U, const orc_geom* g, double cfl, double t_end, int max_steps, double gamma,
+                       double cr, int solver, int limiter, const int* bc, double* ch_inout, double* dts,
+                       double* chs, int* nsteps, double* t_out, double* seconds);
+int orc_num_threads(void);
+
 // Case table: ids 0..3 are the reference's (src/test_cases.cpp:178-269), 4..6 are
 // the unit-box Orszag-Tang, the MHD shock-cloud and Brio-Wu with transmissive
 // y and CFL 0.8.  cfg_out = nx, ny, ng, x0, x1, y0, y1, cfl, t_end, gamma.

@@ -1,0 +1,104 @@
+"""CPU-side checks of the drop-in boundary (no GPU needed).
+
+* libmhd_hip.so loads and exports every function include/mhd_hip.h declares;
+* the case table and the host-side analytic initial states of the product are
+  bitwise those of the oracle (which is pinned to the reference);
+* argument validation and error reporting of the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import mhd_gpu_acc_amd as m
+import oracle
+from tests.util import assert_bitwise
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(mhd_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 19, names
+    L = m.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("case_id", range(7))
+@pytest.mark.parametrize("n", [1, 2])
+def test_case_table_matches_oracle(case_id, n):
+    c = m.get_n_case_config(case_id, n)
+    o = oracle.case_config(case_id, n)
+    got = np.array([c.nx, c.ny, c.ng, c.x_min, c.x_max, c.y_min, c.y_max, c.cfl, c.t_end, c.gamma])
+    exp = np.array([o.nx, o.ny, o.ng, o.x_min, o.x_max, o.y_min, o.y_max, o.cfl, o.t_end, o.gamma])
+    assert_bitwise(got, exp, f"case {case_id}")
+    assert c.bc.as_list() == list(o.bc)
+
+
+def _params():
+    p = m.ICParams()
+    po = oracle.ICParams()
+    vals = dict(gamma=1.83, b_scale=0.77, seed=987654321, n_modes=4, pert_amp=0.05, bump_amp=0.02,
+                bump_sigma=0.05, bump_cx=0.41, bump_cy=0.63)
+    for k, v in vals.items():
+        setattr(p, k, v)
+        setattr(po, k, v)
+    modes = ((0.9, 1, 2, 0.1), (-0.4, -3, 1, 2.2), (0.3, 0, 4, 5.0), (0.7, 2, -2, 1.3))
+    for q, (a, kx, ky, ph) in enumerate(modes):
+        for obj in (p, po):
+            obj.mode_amp[q], obj.mode_kx[q], obj.mode_ky[q], obj.mode_phase[q] = a, kx, ky, ph
+    return p, po
+
+
+@pytest.mark.parametrize("case_id", range(9))
+def test_initial_states_match_oracle(case_id):
+    base = case_id if case_id < 7 else 4
+    c = m.get_n_case_config(base, 1)
+    # keep the grids small: same domain and BCs, fewer cells
+    c.nx, c.ny = (96, 40) if base != 2 and base != 6 else (800, 4)
+    o = oracle.Case(c.nx, c.ny, 2, c.x_min, c.x_max, c.y_min, c.y_max, c.cfl, c.t_end, c.gamma,
+                    tuple(c.bc.as_list()))
+    p, po = _params()
+    U = m.initial_state(case_id, c, p)
+    Uo = oracle.init_state(case_id, o, po)
+    assert_bitwise(U, Uo, f"IC {case_id}")
+
+
+def test_errors_are_reported():
+    cfg = m.get_n_case_config("orszag_tang_unit", 1)
+    cfg.ng = 3
+    with pytest.raises(m.MHDError) as e:
+        m.Run(cfg)
+    assert e.value.code == 1 and "ng" in str(e.value)
+    with pytest.raises(ValueError):
+        m.parse_case_id("no_such_case")
+    with pytest.raises(ValueError):
+        m.get_n_case_config(42, 1)
+    with pytest.raises(ValueError):
+        m.get_n_case_config(2, 0)
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    n = C.c_int()
+    m.lib().mhd_device_count(C.byref(n))
+    if n.value > 0:
+        pytest.skip("a HIP device is present")
+    with pytest.raises(m.MHDError) as e:
+        m.Run(m.get_n_case_config("brio_wu", 1))
+    assert e.value.code == 6
