@@ -57,6 +57,11 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
                           "(make -C mhd_gpu_acc_amd/csrc)")
+    # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64.so with
+    # the same SONAME as /opt/rocm's.  Loading torch's first makes the library
+    # bind to it, so device pointers and streams from torch are valid here (the
+    # other order leaves torch without a usable device).
+    import torch  # noqa: F401
     L = C.CDLL(LIB_PATH)
     L.mhd_last_error.restype = C.c_char_p
     L.mhd_plane_stride.restype = C.c_size_t
