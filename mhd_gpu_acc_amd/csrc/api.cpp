@@ -50,6 +50,10 @@ struct mhd_run {
     double ch = 0.0;                         // c_h, starts at 0 (physics.hpp:41)
     bool acc_valid = false;                  // acc holds the CFL max of U[0] with the current ch
     int seg = 0;
+    int xwg = 0;
+    int ywg = 0;
+    int xgrid = 0;
+    int cus = 1;
     bool profiling = false;
     std::vector<hipEvent_t> events;  // 5 per profiled step
     int prof_steps = 0;
@@ -62,6 +66,7 @@ int check_desc(const mhd_run_desc* d) {
     if (!d) return fail(MHD_ERR_INVALID, "null run description");
     if (d->ng != 2) return fail(MHD_ERR_INVALID, "ng must be 2");
     if (d->nx < 2 || d->ny < 2) return fail(MHD_ERR_INVALID, "nx and ny must be >= 2");
+    if ((long long)(d->nx + 4) * (d->ny + 4) >= (1LL << 31)) return fail(MHD_ERR_INVALID, "grid too large");
     if (d->solver < 0 || d->solver > 3) return fail(MHD_ERR_INVALID, "unknown Riemann solver");
     if (d->limiter < 0 || d->limiter > 1) return fail(MHD_ERR_INVALID, "unknown limiter");
     for (int k = 0; k < 4; ++k)
@@ -112,6 +117,10 @@ int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, doubl
     a.limiter = r->d.limiter;
     std::memcpy(a.bc, r->d.bc, sizeof a.bc);
     a.seg = r->seg;
+    a.xwg = r->xwg;
+    a.ywg = r->ywg;
+    a.xgrid = r->xgrid;
+    a.cus = r->cus;
     a.frame = r->frame;
     a.acc = r->acc;
     a.stream = r->stream();
@@ -158,6 +167,16 @@ int mhd_run_create(const mhd_run_desc* desc, mhd_run** out) {
     r->dy = (desc->y_max - desc->y_min) / static_cast<double>(desc->ny);
     const char* seg = std::getenv("MHD_YSEG");
     r->seg = seg ? std::atoi(seg) : 0;
+    const char* xwg = std::getenv("MHD_XWG");
+    r->xwg = xwg ? std::atoi(xwg) : 0;
+    const char* ywg = std::getenv("MHD_YWG");
+    r->ywg = ywg ? std::atoi(ywg) : 0;
+    const char* xgrid = std::getenv("MHD_XGRID");
+    r->xgrid = xgrid ? std::atoi(xgrid) : 8;  // persistent x sweep, 8 blocks per CU (measured best)
+    int dev = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&r->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || r->cus < 1)
+        r->cus = 1;
     auto cleanup = [&](hipError_t e, const char* what) {
         mhd_run_destroy(r);
         return hip_fail(e, what);
@@ -263,7 +282,7 @@ int mhd_advance(mhd_run* r, double dt) {
     if (!r) return fail(MHD_ERR_INVALID, "null run");
     if (int rc = step_grids(r, r->U[0], r->U[1], r->U[2], dt)) return rc;
     std::swap(r->U[0], r->U[2]);
-    r->acc_valid = true;
+    r->acc_valid = mhd::fused_cfl();
     return MHD_OK;
 }
 

@@ -35,38 +35,39 @@ struct Geom {
 // Field indices in the reference order.
 enum : int { F_RHO = 0, F_MX = 1, F_MY = 2, F_MZ = 3, F_BX = 4, F_BY = 5, F_BZ = 6, F_E = 7, F_PSI = 8 };
 
-template <bool XDIR>
-MHD_DI Cons load_rot(const double* __restrict__ U, long long ps, long long k) {
-    Cons u;
-    u.r = U[k];
-    u.mn = U[(XDIR ? F_MX : F_MY) * ps + k];
-    u.mt = U[(XDIR ? F_MY : F_MX) * ps + k];
-    u.mz = U[F_MZ * ps + k];
-    u.bn = U[(XDIR ? F_BX : F_BY) * ps + k];
-    u.bt = U[(XDIR ? F_BY : F_BX) * ps + k];
-    u.bz = U[F_BZ * ps + k];
-    u.e = U[F_E * ps + k];
-    u.psi = U[F_PSI * ps + k];
-    return u;
-}
+// The nine field planes of a grid, rotated for a sweep direction.  The bases are
+// uniform (scalar registers); lanes add a 32-bit cell offset.
+template <bool XDIR, class T> struct Planes {
+    T* r; T* mn; T* mt; T* mz; T* bn; T* bt; T* bz; T* e; T* psi;
+    MHD_DI Planes(T* U, long long ps)
+        : r(U), mn(U + (XDIR ? F_MX : F_MY) * ps), mt(U + (XDIR ? F_MY : F_MX) * ps), mz(U + F_MZ * ps),
+          bn(U + (XDIR ? F_BX : F_BY) * ps), bt(U + (XDIR ? F_BY : F_BX) * ps), bz(U + F_BZ * ps),
+          e(U + F_E * ps), psi(U + F_PSI * ps) {}
+    MHD_DI Cons load(int k) const { return Cons{r[k], mn[k], mt[k], mz[k], bn[k], bt[k], bz[k], e[k], psi[k]}; }
+    MHD_DI void store(int k, const Cons& u) const {
+        r[k] = u.r; mn[k] = u.mn; mt[k] = u.mt; mz[k] = u.mz; bn[k] = u.bn; bt[k] = u.bt; bz[k] = u.bz;
+        e[k] = u.e; psi[k] = u.psi;
+    }
+};
 
 template <bool XDIR>
-MHD_DI void store_rot(double* __restrict__ U, long long ps, long long k, const Cons& u) {
-    U[k] = u.r;
-    U[(XDIR ? F_MX : F_MY) * ps + k] = u.mn;
-    U[(XDIR ? F_MY : F_MX) * ps + k] = u.mt;
-    U[F_MZ * ps + k] = u.mz;
-    U[(XDIR ? F_BX : F_BY) * ps + k] = u.bn;
-    U[(XDIR ? F_BY : F_BX) * ps + k] = u.bt;
-    U[F_BZ * ps + k] = u.bz;
-    U[F_E * ps + k] = u.e;
-    U[F_PSI * ps + k] = u.psi;
+MHD_DI Cons load_rot(const double* __restrict__ U, long long ps, long long k) {
+    return Planes<XDIR, const double>(U, ps).load((int)k);
+}
+
+// Workgroup barrier that orders LDS only.  __syncthreads() fences global memory
+// too, which makes every lane wait for its in-flight global loads (the next
+// segment's prefetch) before the barrier.
+MHD_DI void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // LDS staging of a 9-double record, field-major so that lane-consecutive
 // accesses hit consecutive banks.
-template <int WG> struct Lds9 {
-    double v[9][WG];
+template <int WG, int NF = 9> struct Lds9 {
+    double v[NF][WG];
     MHD_DI void put(int t, const Cons& u) {
         v[0][t] = u.r; v[1][t] = u.mn; v[2][t] = u.mt; v[3][t] = u.mz; v[4][t] = u.bn;
         v[5][t] = u.bt; v[6][t] = u.bz; v[7][t] = u.e; v[8][t] = u.psi;
@@ -77,54 +78,70 @@ template <int WG> struct Lds9 {
     MHD_DI void put(int t, const Prim& w) {
         v[0][t] = w.r; v[1][t] = w.un; v[2][t] = w.ut; v[3][t] = w.uz; v[4][t] = w.bn;
         v[5][t] = w.bt; v[6][t] = w.bz; v[7][t] = w.p; v[8][t] = w.psi;
+        if constexpr (NF > 9) v[9][t] = w.b2;
     }
     MHD_DI Prim getp(int t) const {
-        return Prim{v[0][t], v[1][t], v[2][t], v[3][t], v[4][t], v[5][t], v[6][t], v[7][t], v[8][t]};
+        double b2 = 0.0;
+        if constexpr (NF > 9) b2 = v[9][t];
+        return Prim{v[0][t], v[1][t], v[2][t], v[3][t], v[4][t], v[5][t], v[6][t], v[7][t], v[8][t], b2};
     }
 };
 
 // ---------------------------------------------------------------------------------------
 // x sweep
 // ---------------------------------------------------------------------------------------
-// Block b covers flattened cells [base + b*(WG-4), +WG) of the interior rows
-// (base = jb*tnx - 2); lanes 2..WG-3 that land on an interior column write Utmp.
+// Segment s covers flattened cells [jb*tnx - 2 + s*(WG-4), +WG) of the interior
+// rows; lanes 2..WG-3 that land on an interior column write Utmp.  A block walks
+// segments blockIdx.x, +gridDim.x, ... and loads the next segment's cells before
+// computing the current one, so global-load latency overlaps the arithmetic.
 template <int SOLVER, int LIMITER, int WG>
 __global__ __launch_bounds__(WG) void xsweep_kernel(const double* __restrict__ Uo, double* __restrict__ Ut, Geom g,
                                                     Phys ph, double dtd, double hdtd,
-                                                    unsigned long long* __restrict__ acc) {
-    __shared__ Lds9<WG> sw;   // primitives, later face fluxes
-    __shared__ Lds9<WG> su;   // lower interface states
-    __shared__ Lds9<WG> sv;   // their primitives
+                                                    unsigned long long* __restrict__ acc, int nseg) {
+    __shared__ Lds9<WG> sw;      // primitives, later face fluxes
+    __shared__ Lds9<WG> su;      // lower interface states
+    __shared__ Lds9<WG, 10> sv;  // their primitives
     const int t = threadIdx.x;
     if (acc && blockIdx.x == 0 && t == 0) *acc = 0ull;  // CFL accumulator for this step's ysweep
-    const long long total = (long long)g.tnx * g.tny;
-    const long long k0 = (long long)NG * g.tnx - 2 + (long long)blockIdx.x * (WG - 4);
-    long long k = k0 + t;
-    const long long kl = k < total ? k : total - 1;  // clamp: tail lanes read valid memory
-    const Cons u = load_rot<true>(Uo, g.ps, kl);
-    const Prim w = to_prim(u, ph);
-    sw.put(t, w);
-    __syncthreads();
-    Face lo, hi;
-    {
-        const int tm = t > 0 ? t - 1 : t;
+    const Planes<true, const double> src(Uo, g.ps);
+    const Planes<true, double> dst(Ut, g.ps);
+    const int last = g.tnx * g.tny - 1;
+    const int kb = NG * g.tnx - 2 + t;
+    int s = blockIdx.x;
+    if (s >= nseg) return;
+    int k = kb + s * (WG - 4);
+    Cons un = src.load(min(k, last));  // clamp: tail lanes read valid memory
+    for (;;) {
+        const Cons u = un;  // the segment loaded one iteration ago
+        const int sn = s + gridDim.x;
+        const int kn = kb + sn * (WG - 4);
+        if (sn < nseg) un = src.load(min(kn, last));
+        const Prim w = to_prim(u, ph);
+        sw.put(t, w);
+        lds_barrier();
+        Face lo, hi;
+        {
+            const int tm = t > 0 ? t - 1 : t;
+            const int tp = t < WG - 1 ? t + 1 : t;
+            reconstruct<LIMITER>(sw.getp(tm), w, sw.getp(tp), hdtd, ph, lo, hi);
+        }
+        su.put(t, lo.u);
+        sv.put(t, lo.w);
+        lds_barrier();
         const int tp = t < WG - 1 ? t + 1 : t;
-        reconstruct<LIMITER>(sw.getp(tm), w, sw.getp(tp), hdtd, ph, lo, hi);
+        const Cons f = riemann<SOLVER, true>(hi.u, hi.w, su.get(tp), sv.getp(tp), ph);  // face (this | next)
+        sw.put(t, f);
+        lds_barrier();
+        if (t >= 2 && t < WG - 2 && k <= last) {
+            const int i = k % g.tnx;
+            const int j = k / g.tnx;
+            if (i >= NG && i < NG + g.nx && j < NG + g.ny) dst.store(k, update(u, dtd, f, sw.get(t - 1)));
+        }
+        if (sn >= nseg) break;
+        lds_barrier();  // sw is rewritten by the next segment
+        s = sn;
+        k = kn;
     }
-    su.put(t, lo.u);
-    sv.put(t, lo.w);
-    __syncthreads();
-    // Face between this cell and the next one.
-    const int tp = t < WG - 1 ? t + 1 : t;
-    const Cons f = riemann<SOLVER, true>(hi.u, hi.w, su.get(tp), sv.getp(tp), ph);
-    sw.put(t, f);
-    __syncthreads();
-    if (t < 2 || t >= WG - 2 || k >= total) return;
-    const int i = (int)(k % g.tnx);
-    const int j = (int)(k / g.tnx);
-    if (i < NG || i >= NG + g.nx || j >= NG + g.ny) return;
-    const Cons n = update(u, dtd, f, sw.get(t - 1));
-    store_rot<true>(Ut, g.ps, k, n);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -157,11 +174,29 @@ MHD_DI void wave_max_to(unsigned long long* acc, double m) {
     if ((threadIdx.x & 63) == 0) atomicMax(acc, (unsigned long long)__double_as_longlong(m));
 }
 
+// Knobs (compile time): MHD_Y_WAVES_PER_EU asks the compiler for that many
+// resident waves per SIMD; MHD_Y_CARRY_PRIM keeps the primitive of the upper
+// interface state across rows instead of recomputing it.
+#ifndef MHD_Y_WAVES_PER_EU
+#define MHD_Y_WAVES_PER_EU 0
+#endif
+#ifndef MHD_Y_CARRY_PRIM
+#define MHD_Y_CARRY_PRIM 1
+#endif
+#if MHD_Y_WAVES_PER_EU > 0
+#define MHD_Y_ATTR __attribute__((amdgpu_waves_per_eu(MHD_Y_WAVES_PER_EU)))
+#else
+#define MHD_Y_ATTR
+#endif
+
 template <int SOLVER, int LIMITER>
-__global__ __launch_bounds__(256) void ysweep_kernel(const double* __restrict__ Ut, double* __restrict__ Un, Geom g,
-                                                     Phys ph, double dtd, double hdtd, double damp, int seg,
-                                                     double* __restrict__ frame,
-                                                     unsigned long long* __restrict__ acc) {
+__global__ __launch_bounds__(256) MHD_Y_ATTR void ysweep_kernel(const double* __restrict__ Ut,
+                                                                double* __restrict__ Un, Geom g, Phys ph,
+                                                                double dtd, double hdtd, double damp, int seg,
+                                                                double* __restrict__ frame,
+                                                                unsigned long long* __restrict__ acc) {
+    const Planes<false, const double> src(Ut, g.ps);
+    const Planes<false, double> dst(Un, g.ps);
     const int i = NG + blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = i < NG + g.nx;
     const int ic = active ? i : NG;  // idle lanes shadow column NG (no stores)
@@ -169,25 +204,31 @@ __global__ __launch_bounds__(256) void ysweep_kernel(const double* __restrict__ 
     const int j1 = min(j0 + seg, NG + g.ny);
     const bool damping = !(damp >= 1.0);  // solver_cpu.cpp:265 skips only when factor >= 1
     double cand = 0.0;
-    Prim w0 = to_prim(load_rot<false>(Ut, g.ps, (long long)(j0 - 2) * g.tnx + ic), ph);
-    Prim w1 = to_prim(load_rot<false>(Ut, g.ps, (long long)(j0 - 1) * g.tnx + ic), ph);
-    Face hprev;  // upper interface state of the previous cell
+    Prim w0 = to_prim(src.load((j0 - 2) * g.tnx + ic), ph);
+    Prim w1 = to_prim(src.load((j0 - 1) * g.tnx + ic), ph);
+    Cons hu;     // upper interface state of the previous cell
+#if MHD_Y_CARRY_PRIM
+    Prim hw;     // and its primitive
+#endif
     Cons fprev;  // face flux below the cell being updated
     for (int r = j0; r <= j1 + 1; ++r) {
-        const Prim w2 = to_prim(load_rot<false>(Ut, g.ps, (long long)r * g.tnx + ic), ph);
+        const Prim w2 = to_prim(src.load(r * g.tnx + ic), ph);
         Face lo, hi;
         reconstruct<LIMITER>(w0, w1, w2, hdtd, ph, lo, hi);  // cell r-1
         if (r >= j0 + 1) {
-            const Cons f = riemann<SOLVER, false>(hprev.u, hprev.w, lo.u, lo.w, ph);  // face (r-2 | r-1)
+#if MHD_Y_CARRY_PRIM
+            const Cons f = riemann<SOLVER, false>(hu, hw, lo.u, lo.w, ph);  // face (r-2 | r-1)
+#else
+            const Cons f = riemann<SOLVER, false>(hu, to_prim(hu, ph), lo.u, lo.w, ph);
+#endif
             if (r >= j0 + 2) {
                 const int j = r - 2;
-                const long long k = (long long)j * g.tnx + ic;
-                const Cons u = load_rot<false>(Ut, g.ps, k);
-                Cons n = update(u, dtd, f, fprev);
+                const int k = j * g.tnx + ic;
+                Cons n = update(src.load(k), dtd, f, fprev);
                 if (active) {
                     if (frame && (j < 2 * NG || j >= g.ny || i < 2 * NG || i >= g.nx)) frame_put(frame, g, i, j, n.psi);
                     if (damping) n.psi *= damp;
-                    store_rot<false>(Un, g.ps, k, n);
+                    dst.store(k, n);
                     if (acc) {
                         Cons x = n;  // back to the x frame for the CFL candidate
                         x.mn = n.mt; x.mt = n.mn; x.bn = n.bt; x.bt = n.bn;
@@ -197,7 +238,10 @@ __global__ __launch_bounds__(256) void ysweep_kernel(const double* __restrict__ 
             }
             fprev = f;
         }
-        hprev = hi;
+        hu = hi.u;
+#if MHD_Y_CARRY_PRIM
+        hw = hi.w;
+#endif
         w0 = w1;
         w1 = w2;
     }
@@ -328,20 +372,34 @@ __global__ __launch_bounds__(256) void cfl_kernel(const double* __restrict__ U, 
 // ===========================================================================================
 #include "launch.hpp"
 
+#ifndef MHD_FUSED_CFL
+#define MHD_FUSED_CFL 1
+#endif
+
 namespace mhd {
+
+bool fused_cfl() { return MHD_FUSED_CFL != 0; }
+
+template <int S, int L, int WG>
+static void launch_x(const StepArgs& a, const Geom& g, const Phys& ph, double dtdx) {
+    const long long flat = (long long)g.tnx * g.ny;
+    const int nseg = (int)((flat + (WG - 4) - 1) / (WG - 4));
+    // xgrid > 0: persistent grid of xgrid blocks per CU; 0: one block per segment
+    const int nb = a.xgrid > 0 ? min(nseg, a.xgrid * a.cus) : nseg;
+    hipLaunchKernelGGL((xsweep_kernel<S, L, WG>), dim3((unsigned)nb), dim3(WG), 0, a.stream, a.Uold, a.Utmp, g, ph,
+                       dtdx, 0.5 * dtdx, a.acc, nseg);
+}
 
 template <int S, int L>
 static hipError_t launch_step_sl(const StepArgs& a) {
     const Geom g{a.nx, a.ny, a.nx + 2 * NG, a.ny + 2 * NG, a.ps};
     Phys ph{a.gamma, a.gamma - 1.0, a.ch, a.ch * a.ch, 0.5 * a.ch, 0.5 / a.ch};
     const double dtdx = a.dt / a.dx, dtdy = a.dt / a.dy;
-    constexpr int WG = 256;
-    const long long flat = (long long)g.tnx * g.ny + 4;
-    const unsigned nbx = (unsigned)((flat + (WG - 4) - 1) / (WG - 4));
     hipError_t e;
     if (a.ev && (e = hipEventRecord(a.ev[0], a.stream)) != hipSuccess) return e;
-    hipLaunchKernelGGL((xsweep_kernel<S, L, WG>), dim3(nbx), dim3(WG), 0, a.stream, a.Uold, a.Utmp, g, ph, dtdx,
-                       0.5 * dtdx, a.acc);
+    if (a.xwg == 64) launch_x<S, L, 64>(a, g, ph, dtdx);
+    else if (a.xwg == 128) launch_x<S, L, 128>(a, g, ph, dtdx);
+    else launch_x<S, L, 256>(a, g, ph, dtdx);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.ev && (e = hipEventRecord(a.ev[1], a.stream)) != hipSuccess) return e;
     const int nbc = 2 * NG * g.ny + 2 * NG * g.tnx;
@@ -349,10 +407,11 @@ static hipError_t launch_step_sl(const StepArgs& a) {
                        a.bc[1], a.bc[2], a.bc[3], (const double*)nullptr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.ev && (e = hipEventRecord(a.ev[2], a.stream)) != hipSuccess) return e;
-    const int seg = a.seg > 0 ? a.seg : 64;
-    dim3 gy((unsigned)((g.nx + 255) / 256), (unsigned)((g.ny + seg - 1) / seg));
-    hipLaunchKernelGGL((ysweep_kernel<S, L>), gy, dim3(256), 0, a.stream, a.Utmp, a.Unew, g, ph, dtdy, 0.5 * dtdy,
-                       a.damp, seg, a.frame, a.acc);
+    const int seg = a.seg > 0 ? a.seg : 32;
+    const int ywg = (a.ywg == 64 || a.ywg == 128) ? a.ywg : 256;
+    dim3 gy((unsigned)((g.nx + ywg - 1) / ywg), (unsigned)((g.ny + seg - 1) / seg));
+    hipLaunchKernelGGL((ysweep_kernel<S, L>), gy, dim3(ywg), 0, a.stream, a.Utmp, a.Unew, g, ph, dtdy, 0.5 * dtdy,
+                       a.damp, seg, a.frame, MHD_FUSED_CFL ? a.acc : nullptr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.ev && (e = hipEventRecord(a.ev[3], a.stream)) != hipSuccess) return e;
     hipLaunchKernelGGL(bc_kernel, dim3((nbc + 255) / 256), dim3(256), 0, a.stream, a.Unew, a.Utmp, g, a.bc[0],

@@ -17,6 +17,10 @@ struct StepArgs {
     int solver, limiter;
     int bc[4];
     int seg;                 // y-sweep rows per lane segment (0 = default)
+    int xwg;                 // x-sweep workgroup width: 64, 128 or 256 (0 = default)
+    int ywg;                 // y-sweep workgroup width: 64, 128 or 256 (0 = default)
+    int xgrid;               // x-sweep blocks per CU (persistent), 0 = one block per segment
+    int cus;                 // compute units of the device
     double* frame;           // undamped psi of the interior frame (2*2*(nx+ny) doubles)
     unsigned long long* acc; // max signal speed accumulator (bit pattern of a double >= 0)
     hipStream_t stream;
@@ -24,6 +28,8 @@ struct StepArgs {
 };
 
 hipError_t launch_step(const StepArgs& a);
+// Whether the y sweep reduces the next step's CFL maximum (else compute_dt launches it).
+bool fused_cfl();
 hipError_t launch_cfl(const double* U, int nx, int ny, long long ps, double gamma, double ch,
                       unsigned long long* acc, hipStream_t s);
 hipError_t launch_bc(double* U, const double* prev, int nx, int ny, long long ps, const int* bc, hipStream_t s);

@@ -25,9 +25,13 @@ namespace mhd {
 struct Cons {
     double r, mn, mt, mz, bn, bt, bz, e, psi;
 };
-// Primitive state, rotated: (rho, u_n, u_t, u_z, B_n, B_t, B_z, p, psi).
+// Primitive state, rotated: (rho, u_n, u_t, u_z, B_n, B_t, B_z, p, psi), plus
+// |B|^2 as cons_to_prim forms it (physics.hpp:256): flux_x/flux_y and
+// fast_speed evaluate the same expression on the same components, so reusing it
+// is bitwise neutral.  Only to_prim fills b2.
 struct Prim {
     double r, un, ut, uz, bn, bt, bz, p, psi;
+    double b2;
 };
 
 // Per-launch physics constants.
@@ -59,6 +63,7 @@ MHD_DI Prim to_prim(const Cons& u, const Phys& ph) {
     const double ke = 0.5 * u.r * (w.un * w.un + w.ut * w.ut + w.uz * w.uz);
     w.p = ph.gm1 * (u.e - ke - 0.5 * b2);
     w.psi = u.psi;
+    w.b2 = b2;
     return w;
 }
 
@@ -82,8 +87,7 @@ MHD_DI Cons to_cons(const Prim& w, const Phys& ph) {
 // physics.hpp:105-164.  `w` must be to_prim(u): the reference recomputes the same
 // expressions inside flux_x/flux_y, so passing them in is bitwise neutral.
 MHD_DI Cons flux(const Cons& u, const Prim& w, const Phys& ph) {
-    const double b2 = u.bn * u.bn + u.bt * u.bt + u.bz * u.bz;
-    const double pt = w.p + 0.5 * b2;
+    const double pt = w.p + 0.5 * w.b2;
     const double bu = u.bn * w.un + u.bt * w.ut + u.bz * w.uz;
     Cons f;
     f.r = u.mn;
@@ -101,7 +105,7 @@ MHD_DI Cons flux(const Cons& u, const Prim& w, const Phys& ph) {
 // physics.hpp:175-190 (fast magnetosonic speed along the normal)
 MHD_DI double fast_speed(const Prim& w, const Phys& ph) {
     const double a2 = ph.gamma * w.p / w.r;
-    const double b2 = (w.bn * w.bn + w.bt * w.bt + w.bz * w.bz) / w.r;
+    const double b2 = w.b2 / w.r;
     const double bn2 = w.bn * w.bn / w.r;
     const double disc = (a2 + b2) * (a2 + b2) - 4.0 * a2 * bn2;
     return sqrt(0.5 * (a2 + b2 + sqrt(disc > 0.0 ? disc : 0.0)));
@@ -176,8 +180,8 @@ MHD_DI Cons hllc(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur,
     const double sr = fmax(vl.un + cfl, vr.un + cfr);
     if (sl >= 0.0) return fl;
     if (sr <= 0.0) return fr;
-    const double bl2 = vl.bn * vl.bn + vl.bt * vl.bt + vl.bz * vl.bz;
-    const double br2 = vr.bn * vr.bn + vr.bt * vr.bt + vr.bz * vr.bz;
+    const double bl2 = vl.b2;  // B_n^2 + B_t^2 + B_w^2 (riemann.hpp:180-181, 306-307)
+    const double br2 = vr.b2;
     const double ptl = vl.p + 0.5 * bl2;
     const double ptr = vr.p + 0.5 * br2;
     const double num = (sr - vr.un) * vr.r * vr.un - (sl - vl.un) * vl.r * vl.un + ptl - ptr;
@@ -237,8 +241,8 @@ MHD_DI Cons hlld(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur,
     const double sl = fmin(vl.un - cfl, vr.un - cfr);
     const double sr = fmax(vl.un + cfl, vr.un + cfr);
     if (fabs(sr - sl) < 1.0e-14) return hll(ul, vl, fl, ur, vr, fr, ph);
-    const double bl2 = vl.bn * vl.bn + vl.bt * vl.bt + vl.bz * vl.bz;
-    const double br2 = vr.bn * vr.bn + vr.bt * vr.bt + vr.bz * vr.bz;
+    const double bl2 = vl.b2;  // B_n^2 + B_t^2 + B_w^2 (riemann.hpp:180-181, 306-307)
+    const double br2 = vr.b2;
     const double ptl = vl.p + 0.5 * bl2;
     const double ptr = vr.p + 0.5 * br2;
     const double num = (sr - vr.un) * vr.r * vr.un - (sl - vl.un) * vl.r * vl.un + ptl - ptr;
@@ -408,13 +412,22 @@ MHD_DI Cons update(const Cons& u, double dtd, const Cons& fp, const Cons& fm) {
 
 // Candidate of compute_dt for one cell (solver_cpu.cpp:291-300): the larger of the
 // x and y signal speeds, or 0 when the cell is skipped.  `u` is in the x frame.
+// a^2 and b^2 are the same expressions for both directions (physics.hpp:176-177,
+// 185-186) and are formed once.
 MHD_DI double cfl_candidate(const Cons& u, const Phys& ph) {
     const Prim w = to_prim(u, ph);
     if (!finite(w.r) || !finite(w.p) || w.r <= 0.0 || w.p <= 0.0) return 0.0;
-    const double sx = fmax(fabs(w.un) + fast_speed(w, ph), ph.ch);
-    Prim wy = w;
-    wy.un = w.ut; wy.ut = w.un; wy.bn = w.bt; wy.bt = w.bn;
-    const double sy = fmax(fabs(wy.un) + fast_speed(wy, ph), ph.ch);
+    const double a2 = ph.gamma * w.p / w.r;
+    const double b2 = w.b2 / w.r;
+    const double bx2 = w.bn * w.bn / w.r;
+    const double by2 = w.bt * w.bt / w.r;
+    const double s = (a2 + b2) * (a2 + b2);
+    const double dx = s - 4.0 * a2 * bx2;
+    const double dy = s - 4.0 * a2 * by2;
+    const double cx = sqrt(0.5 * (a2 + b2 + sqrt(dx > 0.0 ? dx : 0.0)));
+    const double cy = sqrt(0.5 * (a2 + b2 + sqrt(dy > 0.0 ? dy : 0.0)));
+    const double sx = fmax(fabs(w.un) + cx, ph.ch);
+    const double sy = fmax(fabs(w.ut) + cy, ph.ch);
     if (finite(sx) && finite(sy)) return std_max(sx, sy);
     return 0.0;
 }

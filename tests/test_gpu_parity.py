@@ -163,17 +163,20 @@ def test_all_bc_mixes_step_vs_oracle(bc):
     assert_bitwise(r["U"], o["U"], "state incl. ghosts")
 
 
-@pytest.mark.parametrize("seg", [1, 3, 17, 64, 1000])
-def test_launch_shape_independence(seg, monkeypatch):
-    """y-sweep segment length changes the launch grid, never the result."""
-    monkeypatch.setenv("MHD_YSEG", str(seg))
+@pytest.mark.parametrize("knobs", [dict(MHD_YSEG=1), dict(MHD_YSEG=3), dict(MHD_YSEG=17), dict(MHD_YSEG=1000),
+                                   dict(MHD_XWG=64), dict(MHD_XWG=128), dict(MHD_XGRID=1), dict(MHD_XGRID=3),
+                                   dict(MHD_XWG=64, MHD_XGRID=2), dict(MHD_YWG=64), dict(MHD_YWG=128, MHD_YSEG=5)])
+def test_launch_shape_independence(knobs, monkeypatch):
+    """Workgroup widths, persistent grids and segment lengths change the launch, never the result."""
+    for k, v in knobs.items():
+        monkeypatch.setenv(k, str(v))
     cfg = m.get_n_case_config("orszag_tang_unit", 1)
     cfg.nx, cfg.ny = 70, 90
     U0 = m.initial_state(4, cfg)
-    say(f"[gpu] OT 70x90 seg={seg}")
+    say(f"[gpu] OT 70x90 {knobs}")
     r = gpu_run(cfg, U0, 2, 1, 8)
     o = oracle.run_loop(U0, ocase(cfg), 2, 1, 8)
-    assert_bitwise(r["U"], o["U"], f"seg={seg}")
+    assert_bitwise(r["U"], o["U"], f"{knobs}")
 
 
 def test_small_and_odd_grids():
