@@ -52,6 +52,8 @@ def main():
     ap.add_argument("--limiter", default="vanleer", choices=sorted(LIMITERS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-form measurement")
+    ap.add_argument("--batch-steps", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the path shards by y-slabs but no multi-GPU "
@@ -125,6 +127,9 @@ def main():
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL_SWEEP * cells},
     }
 
+    if not args.no_batch:
+        out["batched"] = bench_batch(args.batch_steps)
+
     if not args.no_cpu:
         import oracle
         case = oracle.Case(n, n, 2, cfg.x_min, cfg.x_max, cfg.y_min, cfg.y_max, cfg.cfl, cfg.t_end, cfg.gamma,
@@ -140,6 +145,60 @@ def main():
                                          f"C++/OpenMP restatement of the reference CPU path"}
         out["parity"] = {"dt_sequence_bitwise_vs_cpu": same, "steps_compared": kk}
     print(json.dumps(out), flush=True)
+
+
+def bench_batch(steps: int, seed: int = 20261203):
+    """64 Orszag-Tang-family runs (128^2 x32, 256^2 x16, 512^2 x12, 1024^2 x4; HLLD, van Leer):
+    the same runs stepped one after another with the plain path, then as one batch
+    (autotuned, one graph replay per step).  Only the step loops are timed."""
+    import hashlib
+
+    import numpy as np
+    import torch
+
+    import mhd_gpu_acc_amd as m
+    from mhd_gpu_acc_amd.batch import ot_family_batch
+
+    def h(U):
+        return hashlib.sha256(np.ascontiguousarray(U).tobytes()).hexdigest()
+
+    b, cfgs, params, states = ot_family_batch(seed)
+    cells = sum(c.nx * c.ny for c in cfgs)
+    report = b.tune(2)
+    b.step(1)  # warm-up (graph upload), then re-seed the states
+    for r, U in enumerate(states):
+        b.set_state(r, U)
+        b.set_ch(r, 0.0)
+    b.step(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps, record=False)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    check = [0, 1, 2] + [r for r in range(len(cfgs)) if cfgs[r].nx == 1024][:1]
+    bh = {r: h(b.get_state(r)) for r in check}
+    b.close()
+
+    t_plain = 0.0
+    same = True
+    for r, (c, U0) in enumerate(zip(cfgs, states)):
+        with m.Run(c, m.RiemannSolver.HLLD, m.Limiter.VAN_LEER) as run:
+            run.set_state(U0)
+            run.step_to(1.0, 1)  # warm-up step, then restart from the IC
+            run.set_state(U0)
+            run.ch = 0.0
+            run.synchronize()
+            t0 = time.perf_counter()
+            rr = run.step_to(1.0, steps, record=False)
+            assert rr["code"] == 0 and rr["steps"] == steps, rr
+            run.synchronize()
+            t_plain += time.perf_counter() - t0
+            if r in bh:
+                same = same and (h(run.get_state()) == bh[r])
+    best = report.strip().splitlines()[-1]
+    return {"value": steps * cells / t_batch, "unit": "cell-updates/s", "runs": len(cfgs), "cells_per_step": cells,
+            "steps": steps, "sequential_plain": steps * cells / t_plain, "speedup": t_plain / t_batch,
+            "bitwise_equal_to_plain_runs_checked": same, "runs_checked": check, "tuned": best}
 
 
 if __name__ == "__main__":

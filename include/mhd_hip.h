@@ -152,6 +152,40 @@ int mhd_advance_grids(mhd_run* run, const double* Uold, double* Utmp, double* Un
 /* Synchronise the run's stream (errors from asynchronous launches surface here). */
 int mhd_sync(mhd_run* run);
 
+/* ---- batched form (no counterpart in the reference, which runs one case per
+ *      process with process-global physics, head/physics.hpp:26-43) ------------------
+ * n independent runs -- any grid sizes, gamma, c_r, CFL and BCs; one Riemann solver
+ * and one limiter per batch -- advance together.  Per step, one launch per sweep
+ * covers every run (a persistent grid over a tile list spanning all runs), one
+ * launch fills every run's ghosts, and the whole step is a hipGraph replayed each
+ * step.  Each run keeps its own c_h ratchet, dt from its own CFL maximum and psi
+ * damping, computed with the arithmetic of mhd_compute_dt / mhd_advance, so each
+ * run is bitwise the run mhd_step_to produces for it alone. */
+typedef struct mhd_batch mhd_batch;
+int mhd_batch_create(const mhd_run_desc* descs, int n, mhd_batch** out);
+int mhd_batch_destroy(mhd_batch* batch);
+int mhd_batch_size(const mhd_batch* batch, int* n);
+int mhd_batch_set_state(mhd_batch* batch, int run, const double* src, int layout);
+int mhd_batch_get_state(mhd_batch* batch, int run, double* dst, int layout);
+int mhd_batch_get_ch(const mhd_batch* batch, int run, double* ch);
+int mhd_batch_set_ch(mhd_batch* batch, int run, double ch);
+/* Every run takes nsteps steps (compute_dt + advance).  dts (nullable) receives
+ * nsteps x n values, step-major.  MHD_ERR_NO_SIGNAL if some run has no signal
+ * speed; *done = steps completed by all runs. */
+int mhd_batch_step(mhd_batch* batch, int nsteps, double* dts, int* done);
+/* Autotuner: times every candidate launch shape (x-sweep width and blocks per CU,
+ * y-sweep rows per tile and blocks per CU) by graph replays from a saved copy of
+ * the states, restores the states and keeps the fastest.  report (nullable)
+ * receives the timing table as text. */
+int mhd_batch_tune(mhd_batch* batch, int reps, char* report, size_t report_len);
+int mhd_batch_set_shape(mhd_batch* batch, int xwg, int xgrid, int yseg, int ygrid);
+/* Parameters of the Orszag-Tang family batch: the multiset {128 x32, 256 x16,
+ * 512 x12, 1024 x4} (n = 64) shuffled with the seed, per run gamma = 1.4 + 0.6 u,
+ * b_scale = 0.5 + u and 4 density modes (amplitude 2u - 1, integer |k| <= 4,
+ * k != 0, phase 2 pi u), u from one SplitMix64 stream.  For n != 64 the sizes
+ * cycle through 128, 256, 512.  sizes[n], params[n]. */
+int mhd_batch_plan_ot_family(uint64_t seed, int n, int* sizes, mhd_ic_params* params);
+
 /* ---- measurement (no counterpart in the reference, whose only timer is the
  *      steady_clock around the loop, main_cpu.cpp:191,254-269) --------------------
  * While enabled, every step records HIP events on the run's stream around its

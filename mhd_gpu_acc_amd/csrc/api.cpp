@@ -141,6 +141,10 @@ int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, doubl
 
 }  // namespace
 
+namespace mhd {
+int set_error(int code, const char* msg) { return fail(code, msg); }
+}  // namespace mhd
+
 extern "C" {
 
 const char* mhd_last_error(void) { return g_err.c_str(); }

@@ -1,0 +1,610 @@
+// Batched form: many independent runs (any grid sizes, gamma, c_r, CFL and BCs;
+// one Riemann solver and limiter per batch) advanced together.
+//
+// Per step there is one launch per sweep for the whole batch -- a persistent grid
+// walking a tile list that spans every run (x: 252-cell flattened segments; y:
+// 64-column wave tiles of `seg` rows) -- and one launch per ghost fill.  The step
+// (parameter upload, x sweep, ghost fill, y sweep with fused damping and CFL
+// reduction, ghost fill, read-back of every run's CFL maximum) is captured once as
+// a hipGraph and replayed.
+//
+// Each run keeps its own c_h ratchet, dt and damping factor.  They are computed on
+// the host with exactly the arithmetic of mhd_compute_dt / mhd_advance (the damping
+// factor with the host libm exp, solver_cpu.cpp:264), from the 8-byte maximum the
+// y sweep reduced, so every run of a batch is bitwise the run that mhd_step_to
+// would produce on its own.  The device code is the same as the plain path's
+// (sweeps.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mhd_hip.h"
+#include "launch.hpp"
+#include "sweeps.hpp"
+
+#pragma clang fp contract(off)
+
+namespace mhd {
+
+struct BatchRun {   // constant for the life of the batch
+    double* U[3];   // U[cur] is the state, U[1] the intermediate, U[2 - cur] the next state
+    double* frame;
+    int nx, ny;
+    long long ps;
+    double gamma;
+    int bc[4];
+};
+
+struct BatchStep {  // rewritten by the host before every step
+    double ch, dtdx, dtdy, damp;
+    int cur;        // 0 or 2
+    int pad;
+};
+
+MHD_DI Geom geom_of(const BatchRun& R) { return Geom{R.nx, R.ny, R.nx + 2 * NG, R.ny + 2 * NG, R.ps}; }
+// Same operations as the host builds for a plain launch (kernels.hip launch_step_sl).
+MHD_DI Phys phys_of(double gamma, double ch) { return Phys{gamma, gamma - 1.0, ch, ch * ch, 0.5 * ch, 0.5 / ch}; }
+
+template <int SOLVER, int LIMITER, int WG>
+__global__ __launch_bounds__(WG) void batch_xsweep_kernel(const BatchRun* __restrict__ runs,
+                                                          const BatchStep* __restrict__ steps,
+                                                          const int2* __restrict__ tiles, int ntiles,
+                                                          unsigned long long* __restrict__ acc, int nruns,
+                                                          unsigned int* __restrict__ queue) {
+    __shared__ XLds<WG> L;
+    const int t = threadIdx.x;
+    if (blockIdx.x == 0) {
+        for (int r = t; r < nruns; r += WG) acc[r] = 0ull;  // CFL maxima of this step's y sweep
+        if (t == 0) *queue = 0u;                             // y-sweep work counter
+    }
+    for (int q = blockIdx.x; q < ntiles; q += gridDim.x) {
+        const int2 tl = tiles[q];
+        const BatchRun& R = runs[tl.x];
+        const BatchStep& P = steps[tl.x];
+        const Geom g = geom_of(R);
+        const Phys ph = phys_of(R.gamma, P.ch);
+        const Planes<true, const double> src(R.U[P.cur], g.ps);
+        const Planes<true, double> dst(R.U[1], g.ps);
+        const int k = NG * g.tnx - 2 + tl.y * (WG - 4) + t;
+        const Cons u = src.load(min(k, g.tnx * g.tny - 1));
+        xsweep_segment<SOLVER, LIMITER, WG>(u, k, t, dst, g, ph, P.dtdx, 0.5 * P.dtdx, L);
+        lds_barrier();  // LDS is rewritten by the next tile
+    }
+}
+
+// Tile = (run, 64-column block, first row, rows), one wave each.  Tiles are
+// sorted by decreasing height and handed out by a work counter (zeroed by the x
+// sweep of the same step), so the large tiles run first and the small ones fill
+// the tail of the launch.
+template <int SOLVER, int LIMITER>
+__global__ __launch_bounds__(256) void batch_ysweep_kernel(const BatchRun* __restrict__ runs,
+                                                           const BatchStep* __restrict__ steps,
+                                                           const int4* __restrict__ tiles, int ntiles,
+                                                           unsigned int* __restrict__ queue,
+                                                           unsigned long long* __restrict__ acc) {
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        // Every lane takes part (lane 0 adds 1, the others 0) and lane 0's ticket is
+        // broadcast: no lane-divergent branch, so the exit test is uniform and the
+        // compiler cannot split the loop between lanes.
+        const unsigned got = atomicAdd(queue, lane == 0 ? 1u : 0u);
+        const int q = __builtin_amdgcn_readfirstlane((int)got);
+        if (q >= ntiles) break;
+        const int4 tl = tiles[q];
+        const BatchRun& R = runs[tl.x];
+        const BatchStep& P = steps[tl.x];
+        const Geom g = geom_of(R);
+        const Phys ph = phys_of(R.gamma, P.ch);
+        const Planes<false, const double> src(R.U[1], g.ps);
+        const Planes<false, double> dst(R.U[2 - P.cur], g.ps);
+        const int i = NG + tl.y * 64 + lane;
+        const int j0 = NG + tl.z;
+        const int j1 = j0 + tl.w;
+        const double cand = ysweep_column<SOLVER, LIMITER>(src, dst, g, ph, P.dtdy, 0.5 * P.dtdy, P.damp, R.frame,
+                                                           true, i, i < NG + g.nx, j0, j1);
+        wave_max_to(acc + tl.x, cand);
+    }
+}
+
+// phase 0: ghosts of U* (previous grid = the state); phase 1: ghosts of the next
+// state (previous grid = U*, psi from the undamped frame).
+__global__ __launch_bounds__(256) void batch_bc_kernel(const BatchRun* __restrict__ runs,
+                                                       const BatchStep* __restrict__ steps,
+                                                       const int2* __restrict__ tiles, int phase) {
+    const int2 tl = tiles[blockIdx.x];
+    const BatchRun& R = runs[tl.x];
+    const BatchStep& P = steps[tl.x];
+    const Geom g = geom_of(R);
+    const int t = tl.y * 256 + threadIdx.x;
+    if (t >= ghost_count(g)) return;
+    if (phase == 0) ghost_fill(R.U[1], R.U[P.cur], g, R.bc, nullptr, t);
+    else ghost_fill(R.U[2 - P.cur], R.U[1], g, R.bc, R.frame, t);
+}
+
+// compute_dt for states not produced by a batched y sweep.  Tile = (run, chunk).
+__global__ __launch_bounds__(256) void batch_cfl_kernel(const BatchRun* __restrict__ runs,
+                                                        const BatchStep* __restrict__ steps,
+                                                        const int2* __restrict__ tiles,
+                                                        unsigned long long* __restrict__ acc) {
+    const int2 tl = tiles[blockIdx.x];
+    const BatchRun& R = runs[tl.x];
+    const BatchStep& P = steps[tl.x];
+    const Geom g = geom_of(R);
+    const Phys ph = phys_of(R.gamma, P.ch);
+    const int q = tl.y * 256 + threadIdx.x;
+    double m = 0.0;
+    if (q < g.nx * g.ny) {
+        const int j = NG + q / g.nx, i = NG + q % g.nx;
+        m = cfl_candidate(Planes<true, const double>(R.U[P.cur], g.ps).load(j * g.tnx + i), ph);
+    }
+    wave_max_to(acc + tl.x, m);
+}
+
+}  // namespace mhd
+
+using namespace mhd;
+
+struct mhd_batch {
+    int n = 0, solver = 0, limiter = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int cus = 1;
+    std::vector<mhd_run_desc> d;
+    std::vector<double> dx, dy, ch;
+    std::vector<BatchRun> runs;
+    BatchRun* d_runs = nullptr;
+    BatchStep* d_steps = nullptr;
+    BatchStep* h_steps = nullptr;        // pinned
+    unsigned long long* d_acc = nullptr;
+    unsigned long long* h_acc = nullptr; // pinned
+    int cur = 0;
+    bool acc_valid = false;
+    // launch shape (autotuned)
+    int xwg = 256, xgrid = 8, yseg = 64, ygrid = 8;  // yseg: tall tile height; tails use yseg/4
+    unsigned int* d_queue = nullptr;
+    int2* d_xt = nullptr;  int nxt = 0;
+    int4* d_yt = nullptr;  int nyt = 0;
+    int2* d_bt = nullptr;  int nbt = 0;
+    int2* d_ct = nullptr;  int nct = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+#define B_HIP(call)                                                                            \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) return set_error(MHD_ERR_HIP, (std::string(#call) + ": " +       \
+                                                             hipGetErrorString(e_)).c_str()); \
+    } while (0)
+
+template <class T> int upload(T** dptr, const std::vector<T>& v) {
+    if (*dptr) hipFree(*dptr);
+    *dptr = nullptr;
+    if (v.empty()) return MHD_OK;
+    B_HIP(hipMalloc(dptr, v.size() * sizeof(T)));
+    B_HIP(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MHD_OK;
+}
+
+int build_tiles(mhd_batch* b) {
+    std::vector<int2> xt, bt, ct;
+    std::vector<int4> yt;
+    for (int r = 0; r < b->n; ++r) {
+        const int nx = b->d[r].nx, ny = b->d[r].ny, tnx = nx + 4;
+        const long long flat = (long long)tnx * ny;
+        const int nseg = (int)((flat + (b->xwg - 4) - 1) / (b->xwg - 4));
+        for (int s = 0; s < nseg; ++s) xt.push_back(make_int2(r, s));
+        // Each 64-column block: tall tiles of yseg rows over about 3/4 of the
+        // column, the rest in tiles of yseg/4 rows (at least 4).
+        const int ncb = (nx + 63) / 64;
+        const int big = b->yseg, small = std::max(4, b->yseg / 4);
+        const int nbig = std::min(ny / big, (3 * ny) / (4 * big));
+        for (int cb = 0; cb < ncb; ++cb) {
+            int j = 0;
+            for (int k = 0; k < nbig; ++k, j += big) yt.push_back(make_int4(r, cb, j, big));
+            for (; j < ny; j += small) yt.push_back(make_int4(r, cb, j, std::min(small, ny - j)));
+        }
+        const int ng = 2 * 2 * ny + 2 * 2 * tnx;
+        for (int c = 0; c < (ng + 255) / 256; ++c) bt.push_back(make_int2(r, c));
+        for (int c = 0; c < (nx * ny + 255) / 256; ++c) ct.push_back(make_int2(r, c));
+    }
+    std::stable_sort(yt.begin(), yt.end(), [](const int4& a, const int4& c) { return a.w > c.w; });
+    b->nxt = (int)xt.size();
+    b->nyt = (int)yt.size();
+    b->nbt = (int)bt.size();
+    b->nct = (int)ct.size();
+    if (int rc = upload(&b->d_xt, xt)) return rc;
+    if (int rc = upload(&b->d_yt, yt)) return rc;
+    if (int rc = upload(&b->d_bt, bt)) return rc;
+    return upload(&b->d_ct, ct);
+}
+
+template <int S, int L, int WG> void launch_bx(mhd_batch* b) {
+    const int nb = std::min(b->nxt, b->xgrid * b->cus);
+    hipLaunchKernelGGL((batch_xsweep_kernel<S, L, WG>), dim3(std::max(nb, 1)), dim3(WG), 0, b->stream, b->d_runs,
+                       b->d_steps, b->d_xt, b->nxt, b->d_acc, b->n, b->d_queue);
+}
+
+template <int S, int L> hipError_t launch_batch_step_sl(mhd_batch* b) {
+    if (b->xwg == 128) launch_bx<S, L, 128>(b);
+    else launch_bx<S, L, 256>(b);
+    hipLaunchKernelGGL(batch_bc_kernel, dim3(b->nbt), dim3(256), 0, b->stream, b->d_runs, b->d_steps, b->d_bt, 0);
+    const int wpb = 4;
+    const int nb = std::min((b->nyt + wpb - 1) / wpb, b->ygrid * b->cus);
+    hipLaunchKernelGGL((batch_ysweep_kernel<S, L>), dim3(std::max(nb, 1)), dim3(64 * wpb), 0, b->stream, b->d_runs,
+                       b->d_steps, b->d_yt, b->nyt, b->d_queue, b->d_acc);
+    hipLaunchKernelGGL(batch_bc_kernel, dim3(b->nbt), dim3(256), 0, b->stream, b->d_runs, b->d_steps, b->d_bt, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_step(mhd_batch* b) {
+    switch (b->solver * 2 + b->limiter) {
+        case 0: return launch_batch_step_sl<0, 0>(b);
+        case 1: return launch_batch_step_sl<0, 1>(b);
+        case 2: return launch_batch_step_sl<1, 0>(b);
+        case 3: return launch_batch_step_sl<1, 1>(b);
+        case 4: return launch_batch_step_sl<2, 0>(b);
+        case 5: return launch_batch_step_sl<2, 1>(b);
+        case 6: return launch_batch_step_sl<3, 0>(b);
+        case 7: return launch_batch_step_sl<3, 1>(b);
+    }
+    return hipErrorInvalidValue;
+}
+
+void drop_graph(mhd_batch* b) {
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// One step as a graph: parameter upload, the four launches, CFL maxima read-back.
+int capture_graph(mhd_batch* b) {
+    drop_graph(b);
+    B_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+    hipError_t e = hipMemcpyAsync(b->d_steps, b->h_steps, b->n * sizeof(BatchStep), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = launch_batch_step(b);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->h_acc, b->d_acc, b->n * sizeof(unsigned long long), hipMemcpyDeviceToHost, b->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return set_error(MHD_ERR_HIP, (std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2)).c_str());
+    }
+    b->graph = g;
+    B_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return MHD_OK;
+}
+
+// Every run's CFL maximum of its current state with its current c_h (compute_dt's
+// reduction for states that no batched y sweep produced).
+int initial_cfl(mhd_batch* b) {
+    for (int r = 0; r < b->n; ++r) b->h_steps[r].ch = b->ch[r], b->h_steps[r].cur = b->cur;
+    B_HIP(hipMemcpyAsync(b->d_steps, b->h_steps, b->n * sizeof(BatchStep), hipMemcpyHostToDevice, b->stream));
+    B_HIP(hipMemsetAsync(b->d_acc, 0, b->n * sizeof(unsigned long long), b->stream));
+    hipLaunchKernelGGL(batch_cfl_kernel, dim3(b->nct), dim3(256), 0, b->stream, b->d_runs, b->d_steps, b->d_ct, b->d_acc);
+    B_HIP(hipGetLastError());
+    B_HIP(hipMemcpyAsync(b->h_acc, b->d_acc, b->n * sizeof(unsigned long long), hipMemcpyDeviceToHost, b->stream));
+    B_HIP(hipStreamSynchronize(b->stream));
+    b->acc_valid = true;
+    return MHD_OK;
+}
+
+// compute_dt per run (solver_cpu.cpp:283-312) and the damping factor (:264), from
+// the maxima in h_acc; fills h_steps.  Returns the first run without a signal speed.
+int plan_step(mhd_batch* b, double* dts, int* bad_run) {
+    for (int r = 0; r < b->n; ++r) {
+        double mx;
+        std::memcpy(&mx, &b->h_acc[r], sizeof mx);
+        if (!(mx > 0.0)) {
+            *bad_run = r;
+            return MHD_ERR_NO_SIGNAL;
+        }
+        b->ch[r] = mx;
+        const double dt = b->d[r].cfl * std::min(b->dx[r], b->dy[r]) / mx;
+        BatchStep& P = b->h_steps[r];
+        P.ch = mx;
+        P.dtdx = dt / b->dx[r];
+        P.dtdy = dt / b->dy[r];
+        P.damp = std::exp(-dt * mx / b->d[r].cr);
+        P.cur = b->cur;
+        if (dts) dts[r] = dt;
+    }
+    return MHD_OK;
+}
+
+int check_run(const mhd_batch* b, int r) {
+    if (!b || r < 0 || r >= b->n) return set_error(MHD_ERR_INVALID, "run index out of range");
+    return MHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhd_batch_create(const mhd_run_desc* descs, int n, mhd_batch** out) {
+    if (!out || !descs || n < 1) return set_error(MHD_ERR_INVALID, "need at least one run");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(MHD_ERR_NOT_BUILT, "no HIP device");
+    for (int r = 0; r < n; ++r) {
+        const mhd_run_desc& d = descs[r];
+        if (d.ng != 2 || d.nx < 2 || d.ny < 2 || (long long)(d.nx + 4) * (d.ny + 4) >= (1LL << 31))
+            return set_error(MHD_ERR_INVALID, "bad grid size in batch");
+        if (d.solver != descs[0].solver || d.limiter != descs[0].limiter)
+            return set_error(MHD_ERR_INVALID, "a batch uses one Riemann solver and one limiter");
+        if (d.solver < 0 || d.solver > 3 || d.limiter < 0 || d.limiter > 1)
+            return set_error(MHD_ERR_INVALID, "unknown solver or limiter");
+        for (int k = 0; k < 4; ++k)
+            if (d.bc[k] < 0 || d.bc[k] > 3) return set_error(MHD_ERR_INVALID, "unknown boundary type");
+        if (!(d.x_max > d.x_min) || !(d.y_max > d.y_min)) return set_error(MHD_ERR_INVALID, "empty domain");
+    }
+    mhd_batch* b = new mhd_batch();
+    b->n = n;
+    b->solver = descs[0].solver;
+    b->limiter = descs[0].limiter;
+    b->d.assign(descs, descs + n);
+    b->ch.assign(n, 0.0);
+    int dev = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || b->cus < 1)
+        b->cus = 1;
+    auto bail = [&](int rc) {
+        mhd_batch_destroy(b);
+        return rc;
+    };
+    if (descs[0].stream) {
+        b->stream = static_cast<hipStream_t>(descs[0].stream);
+    } else {
+        if (hipStreamCreate(&b->stream) != hipSuccess) return bail(set_error(MHD_ERR_HIP, "hipStreamCreate"));
+        b->own_stream = true;
+    }
+    b->runs.resize(n);
+    for (int r = 0; r < n; ++r) {
+        const mhd_run_desc& d = descs[r];
+        b->dx.push_back((d.x_max - d.x_min) / static_cast<double>(d.nx));
+        b->dy.push_back((d.y_max - d.y_min) / static_cast<double>(d.ny));
+        BatchRun& R = b->runs[r];
+        R.nx = d.nx;
+        R.ny = d.ny;
+        R.ps = (long long)(d.nx + 4) * (d.ny + 4);
+        R.gamma = d.gamma;
+        std::memcpy(R.bc, d.bc, sizeof R.bc);
+        for (int k = 0; k < 3; ++k) {
+            if (hipMalloc(&R.U[k], 9 * R.ps * sizeof(double)) != hipSuccess)
+                return bail(set_error(MHD_ERR_HIP, "hipMalloc(batch grid)"));
+            hipMemsetAsync(R.U[k], 0, 9 * R.ps * sizeof(double), b->stream);
+        }
+        if (hipMalloc(&R.frame, 4 * (size_t)(d.nx + d.ny) * sizeof(double)) != hipSuccess)
+            return bail(set_error(MHD_ERR_HIP, "hipMalloc(frame)"));
+    }
+    if (int rc = upload(&b->d_runs, b->runs)) return bail(rc);
+    if (hipMalloc(&b->d_steps, n * sizeof(BatchStep)) != hipSuccess ||
+        hipMalloc(&b->d_acc, n * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&b->d_queue, sizeof(unsigned int)) != hipSuccess ||
+        hipHostMalloc(&b->h_steps, n * sizeof(BatchStep), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&b->h_acc, n * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+        return bail(set_error(MHD_ERR_HIP, "batch allocation"));
+    std::memset(b->h_steps, 0, n * sizeof(BatchStep));
+    if (int rc = build_tiles(b)) return bail(rc);
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return bail(set_error(MHD_ERR_HIP, "sync"));
+    *out = b;
+    return MHD_OK;
+}
+
+int mhd_batch_destroy(mhd_batch* b) {
+    if (!b) return MHD_OK;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (BatchRun& R : b->runs) {
+        for (double* p : R.U)
+            if (p) hipFree(p);
+        if (R.frame) hipFree(R.frame);
+    }
+    for (void* p : {(void*)b->d_runs, (void*)b->d_steps, (void*)b->d_acc, (void*)b->d_queue, (void*)b->d_xt, (void*)b->d_yt,
+                    (void*)b->d_bt, (void*)b->d_ct})
+        if (p) hipFree(p);
+    if (b->h_steps) hipHostFree(b->h_steps);
+    if (b->h_acc) hipHostFree(b->h_acc);
+    if (b->own_stream) hipStreamDestroy(b->stream);
+    delete b;
+    return MHD_OK;
+}
+
+int mhd_batch_size(const mhd_batch* b, int* n) {
+    if (!b || !n) return set_error(MHD_ERR_INVALID, "null argument");
+    *n = b->n;
+    return MHD_OK;
+}
+
+int mhd_batch_set_state(mhd_batch* b, int r, const double* src, int layout) {
+    if (int rc = check_run(b, r)) return rc;
+    if (!src) return set_error(MHD_ERR_INVALID, "null state");
+    const BatchRun& R = b->runs[r];
+    const size_t n = (size_t)R.ps;
+    double* dst = R.U[b->cur];
+    if (layout == 0) {
+        std::vector<double> soa(9 * n);
+        for (size_t c = 0; c < n; ++c)
+            for (int f = 0; f < 9; ++f) soa[f * n + c] = src[9 * c + f];
+        B_HIP(hipMemcpy(dst, soa.data(), 9 * n * sizeof(double), hipMemcpyHostToDevice));
+    } else if (layout == 1) {
+        B_HIP(hipMemcpy(dst, src, 9 * n * sizeof(double), hipMemcpyHostToDevice));
+    } else if (layout == 2) {
+        B_HIP(hipMemcpyAsync(dst, src, 9 * n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    } else {
+        return set_error(MHD_ERR_INVALID, "unknown layout");
+    }
+    B_HIP(hipMemcpyAsync(R.U[1], dst, 9 * n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    B_HIP(hipMemcpyAsync(R.U[2 - b->cur], dst, 9 * n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    B_HIP(hipStreamSynchronize(b->stream));
+    b->acc_valid = false;
+    return MHD_OK;
+}
+
+int mhd_batch_get_state(mhd_batch* b, int r, double* dst, int layout) {
+    if (int rc = check_run(b, r)) return rc;
+    if (!dst) return set_error(MHD_ERR_INVALID, "null state");
+    const BatchRun& R = b->runs[r];
+    const size_t n = (size_t)R.ps;
+    B_HIP(hipStreamSynchronize(b->stream));
+    const double* src = R.U[b->cur];
+    if (layout == 0) {
+        std::vector<double> soa(9 * n);
+        B_HIP(hipMemcpy(soa.data(), src, 9 * n * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t c = 0; c < n; ++c)
+            for (int f = 0; f < 9; ++f) dst[9 * c + f] = soa[f * n + c];
+    } else if (layout == 1) {
+        B_HIP(hipMemcpy(dst, src, 9 * n * sizeof(double), hipMemcpyDeviceToHost));
+    } else if (layout == 2) {
+        B_HIP(hipMemcpyAsync(dst, src, 9 * n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+        B_HIP(hipStreamSynchronize(b->stream));
+    } else {
+        return set_error(MHD_ERR_INVALID, "unknown layout");
+    }
+    return MHD_OK;
+}
+
+int mhd_batch_get_ch(const mhd_batch* b, int r, double* ch) {
+    if (int rc = check_run(b, r)) return rc;
+    if (!ch) return set_error(MHD_ERR_INVALID, "null argument");
+    *ch = b->ch[r];
+    return MHD_OK;
+}
+
+int mhd_batch_set_ch(mhd_batch* b, int r, double ch) {
+    if (int rc = check_run(b, r)) return rc;
+    b->ch[r] = ch;
+    b->acc_valid = false;  // the CFL maxima depend on c_h
+    return MHD_OK;
+}
+
+int mhd_batch_step(mhd_batch* b, int nsteps, double* dts, int* done) {
+    if (!b) return set_error(MHD_ERR_INVALID, "null batch");
+    if (done) *done = 0;
+    if (!b->exec)
+        if (int rc = capture_graph(b)) return rc;
+    for (int s = 0; s < nsteps; ++s) {
+        if (!b->acc_valid)
+            if (int rc = initial_cfl(b)) return rc;
+        int bad = -1;
+        if (plan_step(b, dts ? dts + (size_t)s * b->n : nullptr, &bad) != MHD_OK) {
+            char msg[96];
+            std::snprintf(msg, sizeof msg, "compute_dt: non-positive maximum wave speed in run %d.", bad);
+            return set_error(MHD_ERR_NO_SIGNAL, msg);
+        }
+        B_HIP(hipGraphLaunch(b->exec, b->stream));
+        B_HIP(hipStreamSynchronize(b->stream));
+        b->cur = 2 - b->cur;
+        if (done) *done = s + 1;
+    }
+    return MHD_OK;
+}
+
+int mhd_batch_tune(mhd_batch* b, int reps, char* report, size_t report_len) {
+    if (!b) return set_error(MHD_ERR_INVALID, "null batch");
+    if (reps < 1) reps = 3;
+    if (!b->acc_valid)
+        if (int rc = initial_cfl(b)) return rc;
+    // Save every run's state, c_h and CFL maxima; tuning steps are discarded.
+    std::vector<double*> save(b->n, nullptr);
+    const std::vector<double> ch0 = b->ch;
+    std::vector<unsigned long long> acc0(b->h_acc, b->h_acc + b->n);
+    const int cur0 = b->cur;
+    for (int r = 0; r < b->n; ++r) {
+        const size_t bytes = 9 * (size_t)b->runs[r].ps * sizeof(double);
+        B_HIP(hipMalloc(&save[r], bytes));
+        B_HIP(hipMemcpyAsync(save[r], b->runs[r].U[b->cur], bytes, hipMemcpyDeviceToDevice, b->stream));
+    }
+    int bad = -1;
+    if (plan_step(b, nullptr, &bad) != MHD_OK) return set_error(MHD_ERR_NO_SIGNAL, "tune: run without signal speed");
+    const int best0[4] = {b->xwg, b->xgrid, b->yseg, b->ygrid};
+    int best[4] = {best0[0], best0[1], best0[2], best0[3]};
+    double best_ms = 1e30;
+    std::string rep = "xwg xgrid yseg ygrid ms_per_step\n";
+    hipEvent_t e0, e1;
+    B_HIP(hipEventCreate(&e0));
+    B_HIP(hipEventCreate(&e1));
+    const int xwgs[] = {128, 256}, xgrids[] = {8, 16}, ysegs[] = {32, 64, 128}, ygrids[] = {2, 4, 8};
+    int rc = MHD_OK;
+    for (int xw : xwgs)
+        for (int xg : xgrids)
+            for (int ys : ysegs)
+                for (int yg : ygrids) {
+                    b->xwg = xw; b->xgrid = xg; b->yseg = ys; b->ygrid = yg;
+                    if ((rc = build_tiles(b)) || (rc = capture_graph(b))) goto done;
+                    float total = 0.0f;
+                    for (int k = 0; k <= reps; ++k) {
+                        for (int r = 0; r < b->n; ++r) b->h_steps[r].cur = b->cur;
+                        hipEventRecord(e0, b->stream);
+                        hipGraphLaunch(b->exec, b->stream);
+                        hipEventRecord(e1, b->stream);
+                        if (hipStreamSynchronize(b->stream) != hipSuccess) {
+                            rc = set_error(MHD_ERR_HIP, "tune replay");
+                            goto done;
+                        }
+                        float ms = 0.0f;
+                        hipEventElapsedTime(&ms, e0, e1);
+                        if (k > 0) total += ms;  // first replay warms up
+                        b->cur = 2 - b->cur;
+                    }
+                    const double avg = total / reps;
+                    char line[96];
+                    std::snprintf(line, sizeof line, "%d %d %d %d %.4f\n", xw, xg, ys, yg, avg);
+                    rep += line;
+                    if (avg < best_ms) {
+                        best_ms = avg;
+                        best[0] = xw; best[1] = xg; best[2] = ys; best[3] = yg;
+                    }
+                }
+done:
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    // Restore the states and the step bookkeeping, keep the fastest shape.
+    b->cur = cur0;
+    for (int r = 0; r < b->n; ++r) {
+        const size_t bytes = 9 * (size_t)b->runs[r].ps * sizeof(double);
+        hipMemcpyAsync(b->runs[r].U[b->cur], save[r], bytes, hipMemcpyDeviceToDevice, b->stream);
+    }
+    hipStreamSynchronize(b->stream);
+    for (double* p : save)
+        if (p) hipFree(p);
+    b->ch = ch0;
+    std::memcpy(b->h_acc, acc0.data(), b->n * sizeof(unsigned long long));
+    b->acc_valid = true;
+    if (rc != MHD_OK) {
+        std::memcpy(best, best0, sizeof best);
+    }
+    b->xwg = best[0]; b->xgrid = best[1]; b->yseg = best[2]; b->ygrid = best[3];
+    if (int rc2 = build_tiles(b)) return rc2;
+    if (int rc2 = capture_graph(b)) return rc2;
+    char line[128];
+    std::snprintf(line, sizeof line, "best: xwg=%d xgrid=%d yseg=%d ygrid=%d %.4f ms\n", best[0], best[1], best[2],
+                  best[3], best_ms);
+    rep += line;
+    if (report && report_len) {
+        std::strncpy(report, rep.c_str(), report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    return rc;
+}
+
+int mhd_batch_set_shape(mhd_batch* b, int xwg, int xgrid, int yseg, int ygrid) {
+    if (!b) return set_error(MHD_ERR_INVALID, "null batch");
+    if ((xwg != 128 && xwg != 256) || xgrid < 1 || yseg < 1 || ygrid < 1)
+        return set_error(MHD_ERR_INVALID, "bad launch shape");
+    b->xwg = xwg; b->xgrid = xgrid; b->yseg = yseg; b->ygrid = ygrid;
+    if (int rc = build_tiles(b)) return rc;
+    return capture_graph(b);
+}
+
+}  // extern "C"

@@ -183,4 +183,46 @@ int mhd_init_state_host(int case_id, const mhd_case_config* gm, const mhd_ic_par
     return MHD_OK;
 }
 
+int mhd_batch_plan_ot_family(uint64_t seed, int n, int* sizes, mhd_ic_params* params) {
+    if (n < 1 || !sizes || !params) return MHD_ERR_INVALID;
+    uint64_t st = seed;
+    auto uniform = [&]() { return static_cast<double>(splitmix_next(st) >> 11) * 0x1.0p-53; };
+    if (n == 64) {
+        int k = 0;
+        for (int c = 0; c < 32; ++c) sizes[k++] = 128;
+        for (int c = 0; c < 16; ++c) sizes[k++] = 256;
+        for (int c = 0; c < 12; ++c) sizes[k++] = 512;
+        for (int c = 0; c < 4; ++c) sizes[k++] = 1024;
+        for (int i = 63; i >= 1; --i) {  // Fisher-Yates
+            const int j = static_cast<int>(splitmix_next(st) % static_cast<uint64_t>(i + 1));
+            const int t = sizes[i];
+            sizes[i] = sizes[j];
+            sizes[j] = t;
+        }
+    } else {
+        for (int r = 0; r < n; ++r) sizes[r] = 128 << (r % 3);
+    }
+    for (int r = 0; r < n; ++r) {
+        mhd_ic_params& p = params[r];
+        std::memset(&p, 0, sizeof p);
+        p.gamma = 1.4 + 0.6 * uniform();
+        p.b_scale = 0.5 + uniform();
+        p.n_modes = 4;
+        p.pert_amp = 0.05;
+        p.seed = seed;
+        for (int m = 0; m < 4; ++m) {
+            p.mode_amp[m] = 2.0 * uniform() - 1.0;
+            int kx, ky;
+            do {
+                kx = static_cast<int>(splitmix_next(st) % 9) - 4;
+                ky = static_cast<int>(splitmix_next(st) % 9) - 4;
+            } while ((kx == 0 && ky == 0) || kx * kx + ky * ky > 16);
+            p.mode_kx[m] = kx;
+            p.mode_ky[m] = ky;
+            p.mode_phase[m] = 2.0 * kPi * uniform();
+        }
+    }
+    return MHD_OK;
+}
+
 }  // extern "C"

@@ -28,6 +28,8 @@ struct StepArgs {
 };
 
 hipError_t launch_step(const StepArgs& a);
+// Records a failure for mhd_last_error() on this thread and returns code.
+int set_error(int code, const char* msg);
 // Whether the y sweep reduces the next step's CFL maximum (else compute_dt launches it).
 bool fused_cfl();
 hipError_t launch_cfl(const double* U, int nx, int ny, long long ps, double gamma, double ch,

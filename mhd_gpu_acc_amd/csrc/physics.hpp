@@ -171,39 +171,41 @@ MHD_DI Cons hll(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur, 
 // ---------------------------------------------------------------------------------
 // HLLC (riemann.hpp:122-241)
 // ---------------------------------------------------------------------------------
-MHD_DI Cons hllc(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur, const Prim& vr, const Cons& fr,
-                 const Phys& ph) {
-    if (!prim_physical(vl) || !prim_physical(vr)) return hll(ul, vl, fl, ur, vr, fr, ph);
+// Returns false when the reference falls back to HLL (riemann.hpp:131-133, 188,
+// 194-197); otherwise f is the flux.  Only the physical flux of the side the
+// interface lies on is formed (FL for S_M >= 0 or S_L >= 0, FR otherwise), which
+// is the one the reference's region selection returns or builds on.
+MHD_DI bool hllc(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr, const Phys& ph, Cons& f) {
+    if (!prim_physical(vl) || !prim_physical(vr)) return false;
     const double cfl = fast_speed(vl, ph);
     const double cfr = fast_speed(vr, ph);
     const double sl = fmin(vl.un - cfl, vr.un - cfr);
     const double sr = fmax(vl.un + cfl, vr.un + cfr);
-    if (sl >= 0.0) return fl;
-    if (sr <= 0.0) return fr;
-    const double bl2 = vl.b2;  // B_n^2 + B_t^2 + B_w^2 (riemann.hpp:180-181, 306-307)
+    if (sl >= 0.0) { f = flux(ul, vl, ph); return true; }
+    if (sr <= 0.0) { f = flux(ur, vr, ph); return true; }
+    const double bl2 = vl.b2;  // B_n^2 + B_t^2 + B_w^2 (riemann.hpp:180-181)
     const double br2 = vr.b2;
     const double ptl = vl.p + 0.5 * bl2;
     const double ptr = vr.p + 0.5 * br2;
     const double num = (sr - vr.un) * vr.r * vr.un - (sl - vl.un) * vl.r * vl.un + ptl - ptr;
     const double den = (sr - vr.un) * vr.r - (sl - vl.un) * vl.r;
-    if (fabs(den) < 1.0e-14) return hll(ul, vl, fl, ur, vr, fr, ph);
+    if (fabs(den) < 1.0e-14) return false;
     const double sm = num / den;
     const double rls = vl.r * (sl - vl.un) / (sl - sm);
     const double rrs = vr.r * (sr - vr.un) / (sr - sm);
-    if (rls <= 0.0 || rrs <= 0.0 || !finite(rls) || !finite(rrs)) return hll(ul, vl, fl, ur, vr, fr, ph);
+    if (rls <= 0.0 || rrs <= 0.0 || !finite(rls) || !finite(rrs)) return false;
     const double psim = 0.5 * (vl.psi + vr.psi) - ph.half_ch * (vr.bn - vl.bn);
-    if (sm >= 0.0) {
-        const double ptls = ptl + vl.r * (sl - vl.un) * (sm - vl.un);
-        const double bu = vl.bn * vl.un + vl.bt * vl.ut + vl.bz * vl.uz;
-        const double bus = vl.bn * sm + vl.bt * vl.ut + vl.bz * vl.uz;
-        const double es = ((sl - vl.un) * ul.e - ptl * vl.un + ptls * sm + vl.bn * (bus - bu)) / (sl - sm);
-        return fan(fl, sl, star(rls, sm, vl.ut, vl.uz, vl.bn, vl.bt, vl.bz, es, psim), ul);
-    }
-    const double ptrs = ptr + vr.r * (sr - vr.un) * (sm - vr.un);
-    const double bu = vr.bn * vr.un + vr.bt * vr.ut + vr.bz * vr.uz;
-    const double bus = vr.bn * sm + vr.bt * vr.ut + vr.bz * vr.uz;
-    const double es = ((sr - vr.un) * ur.e - ptr * vr.un + ptrs * sm + vr.bn * (bus - bu)) / (sr - sm);
-    return fan(fr, sr, star(rrs, sm, vr.ut, vr.uz, vr.bn, vr.bt, vr.bz, es, psim), ur);
+    const bool left = sm >= 0.0;
+    const Cons& u = left ? ul : ur;
+    const Prim& v = left ? vl : vr;
+    const double s = left ? sl : sr;
+    const double pt = left ? ptl : ptr;
+    const double pts = pt + v.r * (s - v.un) * (sm - v.un);
+    const double bu = v.bn * v.un + v.bt * v.ut + v.bz * v.uz;
+    const double bus = v.bn * sm + v.bt * v.ut + v.bz * v.uz;
+    const double es = ((s - v.un) * u.e - pt * v.un + pts * sm + v.bn * (bus - bu)) / (s - sm);
+    f = fan(flux(u, v, ph), s, star(left ? rls : rrs, sm, v.ut, v.uz, v.bn, v.bt, v.bz, es, psim), u);
+    return true;
 }
 
 // ---------------------------------------------------------------------------------
@@ -233,29 +235,31 @@ MHD_DI SideStar side_star(const Cons& u, const Prim& v, double s, double sm, dou
     return o;
 }
 
-MHD_DI Cons hlld(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur, const Prim& vr, const Cons& fr,
-                 const Phys& ph) {
-    if (!prim_physical(vl) || !prim_physical(vr)) return hll(ul, vl, fl, ur, vr, fr, ph);
+// Returns false when the reference falls back to HLL (riemann.hpp:262-264,
+// 301-303, 316-318, 325-328); otherwise f is the flux.  As in hllc, only the
+// physical flux of the interface's side is formed.
+MHD_DI bool hlld(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr, const Phys& ph, Cons& f) {
+    if (!prim_physical(vl) || !prim_physical(vr)) return false;
     const double cfl = fast_speed(vl, ph);
     const double cfr = fast_speed(vr, ph);
     const double sl = fmin(vl.un - cfl, vr.un - cfr);
     const double sr = fmax(vl.un + cfl, vr.un + cfr);
-    if (fabs(sr - sl) < 1.0e-14) return hll(ul, vl, fl, ur, vr, fr, ph);
-    const double bl2 = vl.b2;  // B_n^2 + B_t^2 + B_w^2 (riemann.hpp:180-181, 306-307)
+    if (fabs(sr - sl) < 1.0e-14) return false;
+    const double bl2 = vl.b2;  // B_n^2 + B_t^2 + B_w^2 (riemann.hpp:306-307)
     const double br2 = vr.b2;
     const double ptl = vl.p + 0.5 * bl2;
     const double ptr = vr.p + 0.5 * br2;
     const double num = (sr - vr.un) * vr.r * vr.un - (sl - vl.un) * vl.r * vl.un + ptl - ptr;
     const double den = (sr - vr.un) * vr.r - (sl - vl.un) * vl.r;
-    if (fabs(den) < 1.0e-14) return hll(ul, vl, fl, ur, vr, fr, ph);
+    if (fabs(den) < 1.0e-14) return false;
     const double sm = num / den;
     const double rls = vl.r * (sl - vl.un) / (sl - sm);
     const double rrs = vr.r * (sr - vr.un) / (sr - sm);
-    if (rls <= 0.0 || rrs <= 0.0 || !finite(rls) || !finite(rrs)) return hll(ul, vl, fl, ur, vr, fr, ph);
+    if (rls <= 0.0 || rrs <= 0.0 || !finite(rls) || !finite(rrs)) return false;
     // The outermost regions are decided before any star state is needed
     // (riemann.hpp:448-449); the values they return do not depend on it.
-    if (sl >= 0.0) return fl;
-    if (sr <= 0.0) return fr;
+    if (sl >= 0.0) { f = flux(ul, vl, ph); return true; }
+    if (sr <= 0.0) { f = flux(ur, vr, ph); return true; }
     const double ptstar = ptl + vl.r * (sl - vl.un) * (sm - vl.un);
     const double sql = sqrt(rls);
     const double sqr = sqrt(rrs);
@@ -267,13 +271,16 @@ MHD_DI Cons hlld(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur,
     const bool left = sm >= 0.0;
     const bool outer = left ? (sls >= 0.0) : (srs <= 0.0);
     const Cons& us = left ? ul : ur;
-    const Cons& fs = left ? fl : fr;
+    const Cons fs = flux(us, left ? vl : vr, ph);
     const SideStar& S = left ? L : R;
     const double s = left ? sl : sr;
     const double rs = left ? rls : rrs;
     const double bn = left ? vl.bn : vr.bn;
     const Cons u1 = star(rs, sm, S.ut, S.uw, bn, S.bt, S.bw, S.e, psim);
-    if (outer) return fan(fs, s, u1, us);  // L* or R* region
+    if (outer) {  // L* or R* region
+        f = fan(fs, s, u1, us);
+        return true;
+    }
     // Double-star states (riemann.hpp:376-413); sign of B_n from the left state.
     const double sgn = (vl.bn >= 0.0) ? 1.0 : -1.0;
     double ut2, uw2, bt2, bw2, e2;
@@ -290,8 +297,8 @@ MHD_DI Cons hlld(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur,
     }
     const double s2 = left ? sls : srs;
     const Cons u2 = star(rs, sm, ut2, uw2, bn, bt2, bw2, e2, psim);
-    const Cons f1 = fan(fs, s, u1, us);
-    return fan(f1, s2, u2, u1);  // L** or R** region
+    f = fan(fan(fs, s, u1, us), s2, u2, u1);  // L** or R** region
+    return true;
 }
 
 // ---------------------------------------------------------------------------------
@@ -334,14 +341,24 @@ MHD_DI void glm_overwrite(Cons& f, const Cons& ul, const Cons& ur, const Phys& p
 // F.Bx in the finiteness test (:583-586): B_n on x sweeps, B_t on y sweeps.
 template <int SOLVER, bool XDIR>
 MHD_DI Cons riemann(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr, const Phys& ph) {
-    const Cons fl = flux(ul, vl, ph);
-    const Cons fr = flux(ur, vr, ph);
-    if (SOLVER == 0) return hll(ul, vl, fl, ur, vr, fr, ph);
-    Cons f = (SOLVER == 1) ? hllc(ul, vl, fl, ur, vr, fr, ph)
-           : (SOLVER == 2) ? hlld(ul, vl, fl, ur, vr, fr, ph)
-                           : force(ul, vl, fl, ur, vr, fr, ph);
+    if (SOLVER == 0 || SOLVER == 3) {
+        const Cons fl = flux(ul, vl, ph);
+        const Cons fr = flux(ur, vr, ph);
+        if (SOLVER == 0) return hll(ul, vl, fl, ur, vr, fr, ph);
+        Cons f = force(ul, vl, fl, ur, vr, fr, ph);
+        const double fbx = XDIR ? f.bn : f.bt;
+        if (!finite(f.r) || !finite(f.e) || !finite(fbx) || !finite(f.psi)) return hll(ul, vl, fl, ur, vr, fr, ph);
+        glm_overwrite(f, ul, ur, ph);
+        return f;
+    }
+    Cons f;
+    const bool ok = (SOLVER == 1) ? hllc(ul, vl, ur, vr, ph, f) : hlld(ul, vl, ur, vr, ph, f);
+    // The solvers' own HLL fallback is an ordinary result of hllc_flux/hlld_flux:
+    // it passes the finiteness test and gets the GLM overwrite like any other.
+    if (!ok) f = hll(ul, vl, flux(ul, vl, ph), ur, vr, flux(ur, vr, ph), ph);
     const double fbx = XDIR ? f.bn : f.bt;
-    if (!finite(f.r) || !finite(f.e) || !finite(fbx) || !finite(f.psi)) return hll(ul, vl, fl, ur, vr, fr, ph);
+    if (!finite(f.r) || !finite(f.e) || !finite(fbx) || !finite(f.psi))
+        return ok ? hll(ul, vl, flux(ul, vl, ph), ur, vr, flux(ur, vr, ph), ph) : f;
     glm_overwrite(f, ul, ur, ph);
     return f;
 }

@@ -1,0 +1,160 @@
+"""Batched form: each run of a batch is bitwise the run the plain path produces
+for it alone (dt sequence and state), before and after autotuning; the OT-family
+plan matches an independent SplitMix64 restatement."""
+from __future__ import annotations
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import mhd_gpu_acc_amd as m
+from mhd_gpu_acc_amd.batch import Batch, ot_family_batch, plan_ot_family
+from tests.util import assert_bitwise
+
+MASK = (1 << 64) - 1
+
+
+def splitmix(state):
+    state = (state + 0x9E3779B97F4A7C15) & MASK
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+    return state, z ^ (z >> 31)
+
+
+def plan_py(seed, n=64):
+    st = seed
+    sizes = [128] * 32 + [256] * 16 + [512] * 12 + [1024] * 4
+    for i in range(63, 0, -1):
+        st, z = splitmix(st)
+        j = z % (i + 1)
+        sizes[i], sizes[j] = sizes[j], sizes[i]
+
+    def uni():
+        nonlocal st
+        st, z = splitmix(st)
+        return (z >> 11) * 2.0 ** -53
+
+    out = []
+    for _ in range(n):
+        g = 1.4 + 0.6 * uni()
+        b = 0.5 + uni()
+        modes = []
+        for _ in range(4):
+            a = 2.0 * uni() - 1.0
+            while True:
+                st, z1 = splitmix(st)
+                st, z2 = splitmix(st)
+                kx, ky = z1 % 9 - 4, z2 % 9 - 4
+                if not ((kx == 0 and ky == 0) or kx * kx + ky * ky > 16):
+                    break
+            modes.append((a, kx, ky, 2.0 * math.pi * uni()))
+        out.append((g, b, modes))
+    return sizes, out
+
+
+def test_ot_family_plan_matches_restatement():
+    sizes, params = plan_ot_family(20261203)
+    psizes, pparams = plan_py(20261203)
+    assert sizes == psizes
+    assert sorted(sizes) == sorted([128] * 32 + [256] * 16 + [512] * 12 + [1024] * 4)
+    for p, (g, b, modes) in zip(params, pparams):
+        assert p.gamma == g and p.b_scale == b and p.n_modes == 4 and p.pert_amp == 0.05
+        assert 1.4 <= p.gamma < 2.0 and 0.5 <= p.b_scale < 1.5
+        for q, (a, kx, ky, ph) in enumerate(modes):
+            assert (p.mode_amp[q], p.mode_kx[q], p.mode_ky[q], p.mode_phase[q]) == (a, kx, ky, ph)
+            assert 0 < kx * kx + ky * ky <= 16
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _mixed_runs():
+    """Six runs of different sizes, gammas, CFLs and BCs (Dirichlet and reflecting included)."""
+    specs = [(16, 16, 1.4, 0.8, (0, 0, 0, 0)), (40, 24, 5 / 3, 0.6, (1, 1, 0, 0)), (64, 64, 2.0, 0.8, (2, 2, 0, 0)),
+             (70, 30, 1.6, 0.5, (0, 0, 2, 1)), (33, 80, 1.9, 0.8, (3, 1, 2, 3)), (130, 66, 5 / 3, 0.7, (0, 0, 0, 0))]
+    cfgs, states = [], []
+    for k, (nx, ny, g, cfl, bc) in enumerate(specs):
+        c = m.CaseConfig(nx, ny, 2, 0.0, 1.0, 0.0, 1.0, cfl, 1.0, g, m.BoundaryConfig(*[m.BoundaryType(b) for b in bc]))
+        p = m.ICParams()
+        p.gamma, p.b_scale, p.n_modes, p.pert_amp = g, 0.6 + 0.15 * k, 4, 0.05
+        for q in range(4):
+            p.mode_amp[q], p.mode_kx[q], p.mode_ky[q], p.mode_phase[q] = 0.3 + 0.1 * q, q - 1, 2 - q, 0.7 * q + k
+        cfgs.append(c)
+        states.append(m.initial_state(m.CaseId.OrszagTangFamily, c, p))
+    return cfgs, states
+
+
+def _plain(cfg, U0, steps, solver, limiter):
+    with m.Run(cfg, solver, limiter) as run:
+        run.set_state(U0)
+        r = run.step_to(1.0, steps)
+        assert r["steps"] == steps, r
+        return r["dts"], run.get_state()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("solver,limiter", [(2, 1), (0, 0), (1, 1), (3, 0)])
+def test_batch_equals_plain_runs(solver, limiter):
+    cfgs, states = _mixed_runs()
+    say(f"[gpu] batch of {len(cfgs)} mixed runs solver={solver} limiter={limiter}")
+    with Batch(cfgs, solver, limiter) as b:
+        for r, U in enumerate(states):
+            b.set_state(r, U)
+        dts = b.step(7)
+        dts2 = b.step(5)
+        finals = [b.get_state(r) for r in range(len(cfgs))]
+    for r, (c, U0) in enumerate(zip(cfgs, states)):
+        pdts, pU = _plain(c, U0, 12, solver, limiter)
+        assert_bitwise(np.concatenate([dts[:, r], dts2[:, r]]), pdts, f"run {r} dt")
+        assert_bitwise(finals[r], pU, f"run {r} state")
+
+
+@pytest.mark.gpu
+def test_batch_tuning_keeps_results():
+    cfgs, states = _mixed_runs()
+    say("[gpu] batch tune")
+    with Batch(cfgs, 2, 1) as b:
+        for r, U in enumerate(states):
+            b.set_state(r, U)
+        b.step(2)
+        report = b.tune(1)
+        assert "best:" in report
+        b.step(4)
+        finals = [b.get_state(r) for r in range(len(cfgs))]
+    for r, (c, U0) in enumerate(zip(cfgs, states)):
+        _, pU = _plain(c, U0, 6, 2, 1)
+        assert_bitwise(finals[r], pU, f"run {r} after tuning")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(128, 1, 1, 1), (256, 3, 7, 2), (128, 16, 64, 16)])
+def test_batch_launch_shapes(shape):
+    cfgs, states = _mixed_runs()
+    say(f"[gpu] batch shape {shape}")
+    with Batch(cfgs, 2, 1) as b:
+        b.set_shape(*shape)
+        for r, U in enumerate(states):
+            b.set_state(r, U)
+        b.step(5)
+        finals = [b.get_state(r) for r in range(len(cfgs))]
+    for r, (c, U0) in enumerate(zip(cfgs, states)):
+        _, pU = _plain(c, U0, 5, 2, 1)
+        assert_bitwise(finals[r], pU, f"run {r} shape {shape}")
+
+
+@pytest.mark.gpu
+def test_ot_family_batch_small():
+    """The 64-run OT family at 1/8 scale (16^2 .. 128^2): every run equals its plain run."""
+    say("[gpu] OT family batch 64 runs at 1/8 scale")
+    b, cfgs, params, states = ot_family_batch(20261203, scale=8)
+    with b:
+        b.step(6)
+        finals = [b.get_state(r) for r in range(b.n)]
+    for r in range(0, 64, 7):
+        _, pU = _plain(cfgs[r], states[r], 6, 2, 1)
+        assert_bitwise(finals[r], pU, f"run {r}")
