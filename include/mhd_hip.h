@@ -186,6 +186,32 @@ int mhd_batch_set_shape(mhd_batch* batch, int xwg, int xgrid, int yseg, int ygri
  * cycle through 128, 256, 512.  sizes[n], params[n]. */
 int mhd_batch_plan_ot_family(uint64_t seed, int n, int* sizes, mhd_ic_params* params);
 
+/* ---- discrete adjoint (no counterpart in the reference) ------------------------
+ * A tape differentiates nsteps steps of a run: U_N = step^N(U_0), each step
+ * compute_dt + advance_second_order.  dt and c_h are frozen at the recorded
+ * values and every branch (Riemann-solver region and fallbacks, limiter
+ * selections, physical-state guards) at the one the recorded trajectory takes.
+ * Supported: HLL and HLLD, minmod and van Leer, periodic / transmissive /
+ * reflecting sides (MHD_ERR_INVALID otherwise).  The gradient is with respect to
+ * every value of U_0, ghost cells included. */
+typedef struct mhd_tape mhd_tape;
+/* Device memory: ceil(nsteps/K) checkpoints, 2K+1 segment grids, 2 adjoint grids. */
+int mhd_tape_create(mhd_run* run, int nsteps, int K, mhd_tape** out);
+int mhd_tape_destroy(mhd_tape* tape);
+/* Advances the run nsteps steps from its current state (as mhd_step_to, without
+ * the t_end clip), keeping the state before every K-th step.  dts and chs
+ * (nullable, nsteps each) receive each step's dt and c_h. */
+int mhd_tape_forward(mhd_tape* tape, double* dts, double* chs);
+/* lam_final = dL/dU_N in, lam0 = dL/dU_0 out, both in an mhd_set_state layout.
+ * Recomputes each K-step segment from its checkpoint, then runs the reverse
+ * steps.  MHD_ERR_INVALID when the recorded trajectory is not finite. */
+int mhd_tape_backward(mhd_tape* tape, const double* lam_final, double* lam0, int layout);
+/* One reverse step over caller device grids: Uold a step's input, Utmp its
+ * intermediate U* (ghosts filled), lam = dL/d(step output) in and dL/dUold out;
+ * lam_tmp is scratch (receives dL/dU*). */
+int mhd_adjoint_step_grids(mhd_run* run, const double* Uold, const double* Utmp, double dt, double ch, double* lam,
+                           double* lam_tmp);
+
 /* ---- measurement (no counterpart in the reference, whose only timer is the
  *      steady_clock around the loop, main_cpu.cpp:191,254-269) --------------------
  * While enabled, every step records HIP events on the run's stream around its

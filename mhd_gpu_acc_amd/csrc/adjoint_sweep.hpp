@@ -1,0 +1,109 @@
+// Sweep-level pieces of the discrete adjoint, shared by the device kernels
+// (adjoint.hip) and the host build of the same arithmetic (hostcheck.cpp).
+//
+// The forward step (solver_cpu.cpp:327-440) is, per sweep,
+//   U1[c] = U[c] - dtd (F[c+1/2] - F[c-1/2])      for interior cells c,
+//   F[c+1/2] = riemann(hi[c], lo[c+1]),  (lo[c], hi[c]) = reconstruct(W[c-1], W[c], W[c+1]),
+//   W = to_prim(U),
+// followed by the ghost fill of U1 from its own interior (and, after the y
+// sweep, psi damping of the interior).  Given g[c], the adjoint of U1 at interior
+// cells with the ghost-cell adjoints folded onto their source cells, the adjoint
+// of U along one line of the sweep is
+//   Fb[c+1/2] = dtd (g[c+1] - g[c])
+//   (hib[c], lob[c+1]) = riemann^T(Fb[c+1/2])
+//   (Wmb[c], Wcb[c], Wpb[c]) = reconstruct^T(lob[c], hib[c])
+//   Ub[c] = to_prim^T((Wpb[c-1] + Wcb[c]) + Wmb[c+1]) + [c interior] g[c].
+// Every quantity is a function of its position alone, so a tiled device sweep and
+// a plain host loop produce the same bits.
+#pragma once
+
+#include "adjoint.hpp"
+#include "sweeps.hpp"
+
+#pragma clang fp contract(off)
+
+namespace mhd {
+
+// Roles of the cell at index a along a sweep line with n interior cells; `line`
+// says whether the line itself is an interior row (x) or column (y).
+struct AdjFlags {
+    bool interior;  // updated by the sweep
+    bool face;      // the face (a | a+1) enters an interior update
+    bool rec;       // the cell's interface states enter such a face
+};
+MHD_DI AdjFlags adj_flags(int a, int n, bool line) {
+    AdjFlags f;
+    f.interior = line && a >= NG && a < NG + n;
+    f.face = line && a >= NG - 1 && a < NG + n;
+    f.rec = line && a >= NG - 1 && a < NG + n + 1;
+    return f;
+}
+
+// Adjoint of the face flux (a | a+1): hi = upper interface state of cell a, lo the
+// lower one of cell a+1; gown/gnext the output adjoints of the two cells.
+template <int SOLVER, bool XDIR>
+MHD_DI void adj_face(const Face& hi, const Face& lo, const Cons& gown, const Cons& gnext, double dtd,
+                     const Phys& ph, Cons& hib, Cons& lob) {
+    const Cons fb{dtd * (gnext.r - gown.r),   dtd * (gnext.mn - gown.mn), dtd * (gnext.mt - gown.mt),
+                  dtd * (gnext.mz - gown.mz), dtd * (gnext.bn - gown.bn), dtd * (gnext.bt - gown.bt),
+                  dtd * (gnext.bz - gown.bz), dtd * (gnext.e - gown.e),   dtd * (gnext.psi - gown.psi)};
+    hib = cons_zero();
+    lob = cons_zero();
+    Prim hwb = prim_zero(), lwb = prim_zero();
+    riemann_adj<SOLVER, XDIR>(hi.u, hi.w, lo.u, lo.w, ph, fb, hib, hwb, lob, lwb);
+    to_prim_adj(hi.u, hi.w, hwb, ph, hib);
+    to_prim_adj(lo.u, lo.w, lwb, ph, lob);
+}
+
+// Output adjoint of cell a: pprev = Wpb[a-1], cown = Wcb[a], mnext = Wmb[a+1].
+MHD_DI Cons adj_cell(const Cons& u, const Prim& w, const Prim& pprev, const Prim& cown, const Prim& mnext,
+                     bool interior, const Cons& g, const Phys& ph) {
+    Prim lw;
+    lw.r = (pprev.r + cown.r) + mnext.r;
+    lw.un = (pprev.un + cown.un) + mnext.un;
+    lw.ut = (pprev.ut + cown.ut) + mnext.ut;
+    lw.uz = (pprev.uz + cown.uz) + mnext.uz;
+    lw.bn = (pprev.bn + cown.bn) + mnext.bn;
+    lw.bt = (pprev.bt + cown.bt) + mnext.bt;
+    lw.bz = (pprev.bz + cown.bz) + mnext.bz;
+    lw.p = (pprev.p + cown.p) + mnext.p;
+    lw.psi = (pprev.psi + cown.psi) + mnext.psi;
+    lw.b2 = 0.0;
+    Cons out = cons_zero();
+    to_prim_adj(u, w, lw, ph, out);
+    if (interior) acc(out, g);
+    return out;
+}
+
+// Adjoint of the interior cell (i, j) of a grid whose ghost cells were filled from
+// its own interior (no Dirichlet side): the cell's own adjoint (psi scaled by the
+// damping factor that followed the fill, 1 when none) plus the adjoints of every
+// ghost cell ghost_source() maps onto it, with the reflections' sign flips.
+// lam(i, j) returns the adjoint at a cell in the rotated frame of the sweep.
+template <bool XDIR, class Lam>
+MHD_DI Cons gather_adj(const Lam& lam, const Geom& g, const int* bc, int i, int j, double psi_scale) {
+    Cons v = lam(i, j);
+    v.psi = v.psi * psi_scale;
+    const int li = i - NG, lj = j - NG;
+    if (li >= NG && li < g.nx - NG && lj >= NG && lj < g.ny - NG) return v;
+    // Candidates: ghost columns x {j} and ghost rows, {i} x ghost rows, and corners.
+    const int gx[5] = {i, 0, 1, NG + g.nx, NG + g.nx + 1};
+    const int gy[5] = {j, 0, 1, NG + g.ny, NG + g.ny + 1};
+    for (int a = 0; a < 5; ++a)
+        for (int b = 0; b < 5; ++b) {
+            if (a == 0 && b == 0) continue;
+            const int ci = gx[a], cj = gy[b];
+            const Src s = ghost_source(ci, cj, g, bc);
+            if (s.prev || s.i != i || s.j != j) continue;
+            Cons w = lam(ci, cj);
+            // reference-frame flips: x negates rho*u and Bx, y negates rho*v and By
+            const bool fn = XDIR ? s.flip_x : s.flip_y;  // normal slots
+            const bool ft = XDIR ? s.flip_y : s.flip_x;  // tangential slots
+            if (fn) { w.mn = -w.mn; w.bn = -w.bn; }
+            if (ft) { w.mt = -w.mt; w.bt = -w.bt; }
+            acc(v, w);
+        }
+    return v;
+}
+
+}  // namespace mhd

@@ -18,8 +18,9 @@
 
 #include "../../include/mhd_hip.h"
 #include "launch.hpp"
+#include "run.hpp"
 
-namespace {
+namespace mhd {
 
 thread_local std::string g_err;
 
@@ -30,35 +31,12 @@ int fail(int code, const std::string& msg) {
 int hip_fail(hipError_t e, const char* what) {
     return fail(MHD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
-#define MHD_HIP(call)                                  \
-    do {                                               \
-        hipError_t e_ = (call);                        \
-        if (e_ != hipSuccess) return hip_fail(e_, #call); \
-    } while (0)
 
-}  // namespace
+}  // namespace mhd
 
-struct mhd_run {
-    mhd_run_desc d;
-    int tnx, tny;
-    size_t cells;  // (nx+4)(ny+4): plane stride of every grid
-    double dx, dy;
-    double* U[3] = {nullptr, nullptr, nullptr};  // current, intermediate (U*), next
-    double* frame = nullptr;
-    unsigned long long* acc = nullptr;
-    unsigned long long* host_acc = nullptr;  // pinned
-    double ch = 0.0;                         // c_h, starts at 0 (physics.hpp:41)
-    bool acc_valid = false;                  // acc holds the CFL max of U[0] with the current ch
-    int seg = 0;
-    int xwg = 0;
-    int ywg = 0;
-    int xgrid = 0;
-    int cus = 1;
-    bool profiling = false;
-    std::vector<hipEvent_t> events;  // 5 per profiled step
-    int prof_steps = 0;
-    hipStream_t stream() const { return static_cast<hipStream_t>(d.stream); }
-};
+using mhd::fail;
+using mhd::g_err;
+using mhd::hip_fail;
 
 namespace {
 
@@ -83,6 +61,10 @@ int read_acc(mhd_run* r, double* out) {
     std::memcpy(out, &bits, sizeof bits);
     return MHD_OK;
 }
+
+}  // namespace
+
+namespace mhd {
 
 // compute_dt on a grid: reduction on the device, ratchet and dt on the host.
 int dt_from_state(mhd_run* r, const double* U, bool reuse, double* dt) {
@@ -139,7 +121,10 @@ int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, doubl
     return MHD_OK;
 }
 
-}  // namespace
+}  // namespace mhd
+
+using mhd::dt_from_state;
+using mhd::step_grids;
 
 namespace mhd {
 int set_error(int code, const char* msg) { return fail(code, msg); }

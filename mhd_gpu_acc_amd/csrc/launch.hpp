@@ -28,6 +28,26 @@ struct StepArgs {
 };
 
 hipError_t launch_step(const StepArgs& a);
+
+// One reverse step (adjoint.hip): lam holds the adjoint of the step's output on
+// entry and of its input (Uold) on return; lam_tmp receives the adjoint of Utmp.
+struct AdjArgs {
+    const double* Uold;      // step input
+    const double* Utmp;      // its x-swept intermediate (ghosts filled)
+    double* lam;
+    double* lam_tmp;         // ghost columns must be zero
+    int nx, ny;
+    long long ps;
+    double dx, dy, dt;
+    double gamma, ch;
+    double damp;             // the step's psi damping factor
+    int solver, limiter;     // solver 0 (HLL) or 2 (HLLD)
+    int bc[4];               // no Dirichlet side
+    hipStream_t stream;
+};
+hipError_t launch_adjoint(const AdjArgs& a);
+// Counts the non-finite values of the 9 planes of a grid into *count (device).
+hipError_t launch_count_nonfinite(const double* U, long long n, long long ps, unsigned int* count, hipStream_t s);
 // Records a failure for mhd_last_error() on this thread and returns code.
 int set_error(int code, const char* msg);
 // Whether the y sweep reduces the next step's CFL maximum (else compute_dt launches it).

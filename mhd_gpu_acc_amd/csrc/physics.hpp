@@ -19,7 +19,8 @@
 
 namespace mhd {
 
-#define MHD_DI __device__ __forceinline__
+// Host and device: the host build backs the self-check entry points (hostcheck.cpp).
+#define MHD_DI __host__ __device__ __forceinline__
 
 // Conserved state, rotated: (rho, m_n, m_t, m_z, B_n, B_t, B_z, E, psi).
 struct Cons {

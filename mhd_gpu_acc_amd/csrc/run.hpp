@@ -1,0 +1,50 @@
+// Private host-side state of a run handle (mhd_run of the C ABI), shared by the
+// host layer (api.cpp) and the adjoint driver (tape.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mhd_hip.h"
+
+struct mhd_run {
+    mhd_run_desc d;
+    int tnx, tny;
+    size_t cells;  // (nx+4)(ny+4): plane stride of every grid
+    double dx, dy;
+    double* U[3] = {nullptr, nullptr, nullptr};  // current, intermediate (U*), next
+    double* frame = nullptr;
+    unsigned long long* acc = nullptr;
+    unsigned long long* host_acc = nullptr;  // pinned
+    double ch = 0.0;                         // c_h, starts at 0 (physics.hpp:41)
+    bool acc_valid = false;                  // acc holds the CFL max of U[0] with the current ch
+    int seg = 0;
+    int xwg = 0;
+    int ywg = 0;
+    int xgrid = 0;
+    int cus = 1;
+    bool profiling = false;
+    std::vector<hipEvent_t> events;  // 5 per profiled step
+    int prof_steps = 0;
+    hipStream_t stream() const { return static_cast<hipStream_t>(d.stream); }
+};
+
+namespace mhd {
+
+// Record msg for mhd_last_error() and return code.
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+// compute_dt on grid U (reuse: the fused CFL maximum of the last step is current).
+int dt_from_state(mhd_run* r, const double* U, bool reuse, double* dt);
+// One advance_second_order from Uold through Utmp into Unew with the run's c_h.
+int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt);
+
+}  // namespace mhd
+
+#define MHD_HIP(call)                                          \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
+        if (e_ != hipSuccess) return mhd::hip_fail(e_, #call); \
+    } while (0)

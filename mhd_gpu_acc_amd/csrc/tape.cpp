@@ -1,0 +1,247 @@
+// Checkpointed discrete adjoint of a fixed number of steps (mhd_tape_* of the C ABI).
+//
+// Forward: the run advances nsteps steps exactly as mhd_step_to would (compute_dt
+// with the c_h ratchet, then advance), the dt and c_h of every step are recorded,
+// and the state before steps 0, K, 2K, ... is copied to a device checkpoint.
+// Backward: for each segment, last first, the segment is recomputed from its
+// checkpoint with the recorded dt and c_h (the same kernels, so the same bits),
+// keeping every step's input U and intermediate U*; then the reverse steps of
+// adjoint.hip run from the segment's last step to its first.  Device memory:
+// ceil(nsteps/K) checkpoints + (2K+1) segment grids + 2 adjoint grids.
+//
+// The adjoint treats the whole update as differentiable: the reference's revert of
+// a cell whose update is non-finite (solver_cpu.cpp:386-392) cannot occur on a
+// finite trajectory, and the backward checks that the recorded trajectory's first
+// and last states are finite.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "launch.hpp"
+#include "run.hpp"
+
+using mhd::fail;
+
+struct mhd_tape {
+    mhd_run* run = nullptr;
+    int nsteps = 0, K = 0, nck = 0;
+    std::vector<double> dts, chs;
+    double ch_after = 0.0;
+    bool recorded = false;
+    std::vector<double*> ck;   // checkpoints
+    std::vector<double*> S;    // segment states, K+1
+    std::vector<double*> T;    // segment intermediates, K
+    double* lam[2] = {nullptr, nullptr};
+    unsigned int* count = nullptr;
+    unsigned int* host_count = nullptr;  // pinned
+};
+
+namespace {
+
+size_t grid_bytes(const mhd_run* r) { return 9 * r->cells * sizeof(double); }
+
+int alloc_grid(mhd_run* r, double** p) {
+    MHD_HIP(hipMalloc(p, grid_bytes(r)));
+    return MHD_OK;
+}
+
+// Device SoA copy of a state or adjoint given in one of the mhd_set_state layouts.
+int upload(mhd_run* r, const double* src, int layout, double* dst) {
+    const size_t n = r->cells;
+    if (layout == 0) {
+        std::vector<double> soa(9 * n);
+        for (size_t c = 0; c < n; ++c)
+            for (int f = 0; f < 9; ++f) soa[f * n + c] = src[9 * c + f];
+        MHD_HIP(hipMemcpyAsync(dst, soa.data(), grid_bytes(r), hipMemcpyHostToDevice, r->stream()));
+        MHD_HIP(hipStreamSynchronize(r->stream()));
+    } else if (layout == 1) {
+        MHD_HIP(hipMemcpyAsync(dst, src, grid_bytes(r), hipMemcpyHostToDevice, r->stream()));
+        MHD_HIP(hipStreamSynchronize(r->stream()));
+    } else if (layout == 2) {
+        MHD_HIP(hipMemcpyAsync(dst, src, grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
+    } else {
+        return fail(MHD_ERR_INVALID, "unknown layout");
+    }
+    return MHD_OK;
+}
+
+int download(mhd_run* r, const double* src, int layout, double* dst) {
+    const size_t n = r->cells;
+    if (layout == 0) {
+        std::vector<double> soa(9 * n);
+        MHD_HIP(hipMemcpyAsync(soa.data(), src, grid_bytes(r), hipMemcpyDeviceToHost, r->stream()));
+        MHD_HIP(hipStreamSynchronize(r->stream()));
+        for (size_t c = 0; c < n; ++c)
+            for (int f = 0; f < 9; ++f) dst[9 * c + f] = soa[f * n + c];
+    } else if (layout == 1) {
+        MHD_HIP(hipMemcpyAsync(dst, src, grid_bytes(r), hipMemcpyDeviceToHost, r->stream()));
+        MHD_HIP(hipStreamSynchronize(r->stream()));
+    } else if (layout == 2) {
+        MHD_HIP(hipMemcpyAsync(dst, src, grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
+        MHD_HIP(hipStreamSynchronize(r->stream()));
+    } else {
+        return fail(MHD_ERR_INVALID, "unknown layout");
+    }
+    return MHD_OK;
+}
+
+int check_adjoint_run(const mhd_run* r) {
+    if (r->d.solver != MHD_HLL && r->d.solver != MHD_HLLD)
+        return fail(MHD_ERR_INVALID, "adjoint: the Riemann solver must be HLL or HLLD");
+    for (int k = 0; k < 4; ++k)
+        if (r->d.bc[k] == MHD_BC_DIRICHLET) return fail(MHD_ERR_INVALID, "adjoint: Dirichlet boundaries are not supported");
+    return MHD_OK;
+}
+
+int nonfinite(mhd_tape* t, const double* U, unsigned int* out) {
+    mhd_run* r = t->run;
+    MHD_HIP(hipMemsetAsync(t->count, 0, sizeof(unsigned int), r->stream()));
+    MHD_HIP(mhd::launch_count_nonfinite(U, (long long)r->cells, (long long)r->cells, t->count, r->stream()));
+    MHD_HIP(hipMemcpyAsync(t->host_count, t->count, sizeof(unsigned int), hipMemcpyDeviceToHost, r->stream()));
+    MHD_HIP(hipStreamSynchronize(r->stream()));
+    *out = *t->host_count;
+    return MHD_OK;
+}
+
+int reverse_step(mhd_run* r, const double* Uold, const double* Utmp, double dt, double ch, double* lam,
+                 double* lam_tmp) {
+    mhd::AdjArgs a{};
+    a.Uold = Uold;
+    a.Utmp = Utmp;
+    a.lam = lam;
+    a.lam_tmp = lam_tmp;
+    a.nx = r->d.nx;
+    a.ny = r->d.ny;
+    a.ps = (long long)r->cells;
+    a.dx = r->dx;
+    a.dy = r->dy;
+    a.dt = dt;
+    a.gamma = r->d.gamma;
+    a.ch = ch;
+    a.damp = std::exp(-dt * ch / r->d.cr);  // as the forward step (api.cpp step_grids)
+    a.solver = r->d.solver;
+    a.limiter = r->d.limiter;
+    std::memcpy(a.bc, r->d.bc, sizeof a.bc);
+    a.stream = r->stream();
+    MHD_HIP(mhd::launch_adjoint(a));
+    return MHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhd_tape_destroy(mhd_tape* t) {
+    if (!t) return MHD_OK;
+    if (t->run) hipStreamSynchronize(t->run->stream());
+    for (double* p : t->ck) if (p) hipFree(p);
+    for (double* p : t->S) if (p) hipFree(p);
+    for (double* p : t->T) if (p) hipFree(p);
+    for (double* p : t->lam) if (p) hipFree(p);
+    if (t->count) hipFree(t->count);
+    if (t->host_count) hipHostFree(t->host_count);
+    delete t;
+    return MHD_OK;
+}
+
+int mhd_tape_create(mhd_run* r, int nsteps, int K, mhd_tape** out) {
+    if (!out) return fail(MHD_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    if (!r) return fail(MHD_ERR_INVALID, "null run");
+    if (nsteps < 1 || K < 1) return fail(MHD_ERR_INVALID, "nsteps and K must be >= 1");
+    if (int rc = check_adjoint_run(r)) return rc;
+    mhd_tape* t = new mhd_tape();
+    t->run = r;
+    t->nsteps = nsteps;
+    t->K = std::min(K, nsteps);
+    t->nck = (nsteps + t->K - 1) / t->K;
+    t->ck.assign(t->nck, nullptr);
+    t->S.assign(t->K + 1, nullptr);
+    t->T.assign(t->K, nullptr);
+    int rc = MHD_OK;
+    for (auto& p : t->ck) if (!rc) rc = alloc_grid(r, &p);
+    for (auto& p : t->S) if (!rc) rc = alloc_grid(r, &p);
+    for (auto& p : t->T) if (!rc) rc = alloc_grid(r, &p);
+    for (auto& p : t->lam) if (!rc) rc = alloc_grid(r, &p);
+    if (!rc) {
+        hipError_t e = hipMalloc(&t->count, sizeof(unsigned int));
+        if (e == hipSuccess) e = hipHostMalloc(&t->host_count, sizeof(unsigned int), hipHostMallocDefault);
+        // the y pass never writes the ghost columns of the U* adjoint: they stay zero
+        if (e == hipSuccess) e = hipMemsetAsync(t->lam[1], 0, grid_bytes(r), r->stream());
+        if (e == hipSuccess) e = hipStreamSynchronize(r->stream());
+        if (e != hipSuccess) rc = mhd::hip_fail(e, "tape allocation");
+    }
+    if (rc) {
+        mhd_tape_destroy(t);
+        return rc;
+    }
+    *out = t;
+    return MHD_OK;
+}
+
+int mhd_tape_forward(mhd_tape* t, double* dts, double* chs) {
+    if (!t) return fail(MHD_ERR_INVALID, "null tape");
+    mhd_run* r = t->run;
+    t->recorded = false;
+    t->dts.assign(t->nsteps, 0.0);
+    t->chs.assign(t->nsteps, 0.0);
+    for (int s = 0; s < t->nsteps; ++s) {
+        if (s % t->K == 0)
+            MHD_HIP(hipMemcpyAsync(t->ck[s / t->K], r->U[0], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
+        double dt = 0.0;
+        if (int rc = mhd_compute_dt(r, &dt)) return rc;
+        if (!std::isfinite(dt) || dt <= 0.0) return fail(MHD_ERR_DT, "dt non-finite or <= 0");
+        if (int rc = mhd_advance(r, dt)) return rc;
+        t->dts[s] = dt;
+        t->chs[s] = r->ch;
+        if (dts) dts[s] = dt;
+        if (chs) chs[s] = r->ch;
+    }
+    MHD_HIP(hipStreamSynchronize(r->stream()));
+    t->ch_after = r->ch;
+    t->recorded = true;
+    return MHD_OK;
+}
+
+int mhd_tape_backward(mhd_tape* t, const double* lam_final, double* lam0, int layout) {
+    if (!t || !lam_final || !lam0) return fail(MHD_ERR_INVALID, "null argument");
+    if (!t->recorded) return fail(MHD_ERR_INVALID, "tape: no recorded forward");
+    mhd_run* r = t->run;
+    unsigned int bad = 0;
+    if (int rc = nonfinite(t, t->ck[0], &bad)) return rc;
+    if (bad) return fail(MHD_ERR_INVALID, "adjoint: the initial state is not finite");
+    if (int rc = upload(r, lam_final, layout, t->lam[0])) return rc;
+    int rc = MHD_OK;
+    for (int k = t->nck - 1; k >= 0 && !rc; --k) {
+        const int s0 = k * t->K, m = std::min(t->K, t->nsteps - s0);
+        MHD_HIP(hipMemcpyAsync(t->S[0], t->ck[k], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
+        for (int q = 0; q < m && !rc; ++q) {
+            r->ch = t->chs[s0 + q];
+            rc = mhd::step_grids(r, t->S[q], t->T[q], t->S[q + 1], t->dts[s0 + q]);
+        }
+        if (rc) break;
+        if (k == t->nck - 1) {
+            if ((rc = nonfinite(t, t->S[m], &bad))) break;
+            if (bad) { rc = fail(MHD_ERR_INVALID, "adjoint: the recorded trajectory is not finite"); break; }
+        }
+        for (int q = m - 1; q >= 0 && !rc; --q)
+            rc = reverse_step(r, t->S[q], t->T[q], t->dts[s0 + q], t->chs[s0 + q], t->lam[0], t->lam[1]);
+    }
+    r->ch = t->ch_after;
+    r->acc_valid = false;
+    if (rc) return rc;
+    return download(r, t->lam[0], layout, lam0);
+}
+
+int mhd_adjoint_step_grids(mhd_run* r, const double* Uold, const double* Utmp, double dt, double ch, double* lam,
+                           double* lam_tmp) {
+    if (!r || !Uold || !Utmp || !lam || !lam_tmp) return fail(MHD_ERR_INVALID, "null argument");
+    if (int rc = check_adjoint_run(r)) return rc;
+    MHD_HIP(hipMemsetAsync(lam_tmp, 0, grid_bytes(r), r->stream()));
+    return reverse_step(r, Uold, Utmp, dt, ch, lam, lam_tmp);
+}
+
+}  // extern "C"

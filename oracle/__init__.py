@@ -82,6 +82,14 @@ def orc():
         lib.orc_apply_glm_flux.argtypes = [_dp, _dp, _dp, C.c_int, C.c_double]
         lib.orc_reconstruct.argtypes = [_dp, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_double,
                                         C.c_int, _dp, _dp]
+        lib.orc_tan_cons_to_prim.argtypes = [_dp, _dp, C.c_double, _dp, _dp]
+        lib.orc_tan_prim_to_cons.argtypes = [_dp, _dp, C.c_double, _dp, _dp]
+        lib.orc_tan_flux.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, _dp, _dp]
+        lib.orc_tan_fast_speed.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, _dp]
+        lib.orc_tan_limiter.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp]
+        lib.orc_tan_riemann.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp]
+        lib.orc_tan_reconstruct.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int, C.c_double,
+                                            C.c_double, C.c_int, _dp, _dp, _dp, _dp]
         _orc = lib
     return _orc
 
@@ -245,3 +253,55 @@ def tangent_run(U: np.ndarray, dU: np.ndarray, case: Case, dts, chs, solver: int
     orc().orc_tangent_run(ptr(U), ptr(dU), C.byref(g), len(dts), ptr(dts), ptr(chs), C.c_double(cr),
                           C.c_double(case.gamma), solver, limiter, ibuf(bc or case.bc))
     return U, dU
+
+
+# ---------------------------------------------------------------------------------------
+# Point tangents (forward mode, branches frozen at the primal values)
+# ---------------------------------------------------------------------------------------
+def _c9(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(9))
+
+
+def tan_cons_to_prim(U, dU, gamma):
+    V, dV = np.zeros(9), np.zeros(9)
+    orc().orc_tan_cons_to_prim(ptr(_c9(U)), ptr(_c9(dU)), C.c_double(gamma), ptr(V), ptr(dV))
+    return V, dV
+
+
+def tan_prim_to_cons(V, dV, gamma):
+    U, dU = np.zeros(9), np.zeros(9)
+    orc().orc_tan_prim_to_cons(ptr(_c9(V)), ptr(_c9(dV)), C.c_double(gamma), ptr(U), ptr(dU))
+    return U, dU
+
+
+def tan_flux(U, dU, direction, ch, gamma):
+    F, dF = np.zeros(9), np.zeros(9)
+    orc().orc_tan_flux(ptr(_c9(U)), ptr(_c9(dU)), direction, C.c_double(ch), C.c_double(gamma), ptr(F), ptr(dF))
+    return F, dF
+
+
+def tan_fast_speed(V, dV, direction, gamma):
+    c, dc = C.c_double(), C.c_double()
+    orc().orc_tan_fast_speed(ptr(_c9(V)), ptr(_c9(dV)), direction, C.c_double(gamma), C.byref(c), C.byref(dc))
+    return c.value, dc.value
+
+
+def tan_limiter(a, da, b, db, limiter):
+    v, dv = C.c_double(), C.c_double()
+    orc().orc_tan_limiter(a, da, b, db, limiter, C.byref(v), C.byref(dv))
+    return v.value, dv.value
+
+
+def tan_riemann(UL, dUL, UR, dUR, direction, solver, ch, gamma):
+    F, dF = np.zeros(9), np.zeros(9)
+    orc().orc_tan_riemann(ptr(_c9(UL)), ptr(_c9(dUL)), ptr(_c9(UR)), ptr(_c9(dUR)), direction, solver,
+                          C.c_double(ch), C.c_double(gamma), ptr(F), ptr(dF))
+    return F, dF
+
+
+def tan_reconstruct(Wm, dWm, Wc, dWc, Wp, dWp, dtd, direction, ch, gamma, limiter):
+    L, dL, R, dR = (np.zeros(9) for _ in range(4))
+    orc().orc_tan_reconstruct(ptr(_c9(Wm)), ptr(_c9(dWm)), ptr(_c9(Wc)), ptr(_c9(dWc)), ptr(_c9(Wp)),
+                              ptr(_c9(dWp)), C.c_double(dtd), direction, C.c_double(ch), C.c_double(gamma),
+                              limiter, ptr(L), ptr(dL), ptr(R), ptr(dR))
+    return L, dL, R, dR

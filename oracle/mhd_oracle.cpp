@@ -49,9 +49,11 @@ inline double val(double x) { return x; }
 inline double val(Dual x) { return x.v; }
 
 inline double s_sqrt(double x) { return std::sqrt(x); }
+// The derivative is taken as 0 where sqrt is 0 (the clamped discriminant of the
+// fast speed), the convention the hand-written adjoint shares.
 inline Dual s_sqrt(Dual x) {
     const double r = std::sqrt(x.v);
-    return Dual(r, x.d / (2.0 * r));
+    return Dual(r, r > 0.0 ? x.d / (2.0 * r) : 0.0);
 }
 inline double s_fabs(double x) { return std::fabs(x); }
 inline Dual s_fabs(Dual x) { return x.v < 0.0 ? -x : (x.v > 0.0 ? x : Dual(std::fabs(x.v), 0.0)); }
@@ -939,6 +941,51 @@ int orc_init_state(int id, const orc_geom* gm, const orc_ic_params* p, double* U
             store9(s, U + 9 * g.at(i, j));
         }
     return 0;
+}
+
+static V9<Dual> dual9(const double* v, const double* d) {
+    V9<Dual> x;
+    for (int k = 0; k < 9; ++k) x[k] = Dual(v[k], d[k]);
+    return x;
+}
+static void split9(const V9<Dual>& x, double* v, double* d) {
+    for (int k = 0; k < 9; ++k) {
+        v[k] = x[k].v;
+        d[k] = x[k].d;
+    }
+}
+
+void orc_tan_cons_to_prim(const double* U, const double* dU, double gamma, double* V, double* dV) {
+    split9(cons_to_prim(dual9(U, dU), gamma), V, dV);
+}
+void orc_tan_prim_to_cons(const double* V, const double* dV, double gamma, double* U, double* dU) {
+    split9(prim_to_cons(dual9(V, dV), gamma), U, dU);
+}
+void orc_tan_flux(const double* U, const double* dU, int dir, double ch, double gamma, double* F, double* dF) {
+    split9(flux(dual9(U, dU), axis_of(dir), Dual(ch), gamma), F, dF);
+}
+void orc_tan_fast_speed(const double* V, const double* dV, int dir, double gamma, double* c, double* dc) {
+    const Dual r = fast_speed(dual9(V, dV), axis_of(dir), gamma);
+    *c = r.v;
+    *dc = r.d;
+}
+void orc_tan_limiter(double a, double da, double b, double db, int limiter, double* v, double* dv) {
+    const Dual r = limit(Dual(a, da), Dual(b, db), limiter);
+    *v = r.v;
+    *dv = r.d;
+}
+void orc_tan_riemann(const double* UL, const double* dUL, const double* UR, const double* dUR, int dir, int solver,
+                     double ch, double gamma, double* F, double* dF) {
+    split9(riemann(dual9(UL, dUL), dual9(UR, dUR), axis_of(dir), solver, Dual(ch), gamma), F, dF);
+}
+void orc_tan_reconstruct(const double* Wm, const double* dWm, const double* Wc, const double* dWc, const double* Wp,
+                         const double* dWp, double dt_over_d, int dir, double ch, double gamma, int limiter,
+                         double* L, double* dL, double* R, double* dR) {
+    V9<Dual> l, r;
+    reconstruct(dual9(Wm, dWm), dual9(Wc, dWc), dual9(Wp, dWp), Dual(dt_over_d), axis_of(dir), Dual(ch), gamma,
+                limiter, l, r);
+    split9(l, L, dL);
+    split9(r, R, dR);
 }
 
 int orc_tangent_run(double* U, double* dU, const orc_geom* gm, int nsteps, const double* dts, const double* chs,

@@ -113,6 +113,19 @@ int  orc_init_state(int case_id, const orc_geom* g, const orc_ic_params* p, doub
 int orc_tangent_run(double* U, double* dU, const orc_geom* g, int nsteps, const double* dts,
                     const double* chs, double cr, double gamma, int solver, int limiter, const int* bc);
 
+// Point tangents (value and directional derivative, same branch rules) of the
+// functions above, for the dot-product tests of the hand-written adjoint.
+void orc_tan_cons_to_prim(const double* U, const double* dU, double gamma, double* V, double* dV);
+void orc_tan_prim_to_cons(const double* V, const double* dV, double gamma, double* U, double* dU);
+void orc_tan_flux(const double* U, const double* dU, int dir, double ch, double gamma, double* F, double* dF);
+void orc_tan_fast_speed(const double* V, const double* dV, int dir, double gamma, double* c, double* dc);
+void orc_tan_limiter(double a, double da, double b, double db, int limiter, double* v, double* dv);
+void orc_tan_riemann(const double* UL, const double* dUL, const double* UR, const double* dUR, int dir, int solver,
+                     double ch, double gamma, double* F, double* dF);
+void orc_tan_reconstruct(const double* Wm, const double* dWm, const double* Wc, const double* dWc, const double* Wp,
+                         const double* dWp, double dt_over_d, int dir, double ch, double gamma, int limiter,
+                         double* L, double* dL, double* R, double* dR);
+
 #ifdef __cplusplus
 }
 #endif
