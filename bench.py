@@ -54,6 +54,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form measurement")
     ap.add_argument("--batch-steps", type=int, default=40)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the path shards by y-slabs but no multi-GPU "
@@ -129,6 +130,8 @@ def main():
 
     if not args.no_batch:
         out["batched"] = bench_batch(args.batch_steps)
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint()
 
     if not args.no_cpu:
         import oracle
@@ -199,6 +202,101 @@ def bench_batch(steps: int, seed: int = 20261203):
     return {"value": steps * cells / t_batch, "unit": "cell-updates/s", "runs": len(cfgs), "cells_per_step": cells,
             "steps": steps, "sequential_plain": steps * cells / t_plain, "speedup": t_plain / t_batch,
             "bitwise_equal_to_plain_runs_checked": same, "runs_checked": check, "tuned": best}
+
+
+def bench_adjoint(n: int = 512, steps: int = 200, K: int = 20, seed: int = 20261203):
+    """Adjoint workload: unit-box Orszag-Tang n^2, HLLD, van Leer, `steps` steps, checkpoint
+    every K.  Loss 1/2 |rho_N - rho_target|^2 with the target from the same run on the
+    state plus a seeded Gaussian density bump; gradient w.r.t. all 9 initial fields
+    through the torch.autograd.Function.  Cost = (forward with checkpoints + backward)
+    / plain forward, each timed alone with the state resident on the device.  The
+    gradient at 32 seeded probes is compared with the oracle's CPU reference values
+    (tangent model and frozen-branch differences, tests/golden/adjoint_ot512.json) and
+    with plain GPU central differences (dt / c_h frozen, branches free)."""
+    import numpy as np
+    import torch
+
+    import mhd_gpu_acc_amd as m
+    from mhd_gpu_acc_amd.adjoint import adjoint_workload, from_planes, probe_pairs, replay, run_steps, to_planes
+
+    cfg, U0, Ubump = adjoint_workload(n, seed)
+    with m.Run(cfg, m.RiemannSolver.HLLD, m.Limiter.VAN_LEER) as run:
+        run.set_state(Ubump)
+        assert run.step_to(1.0, steps)["steps"] == steps
+        target = to_planes(run.get_state())[0, 2:-2, 2:-2].clone()
+
+        def plain():
+            run.set_state(U0)
+            run.ch = 0.0
+            run.synchronize()
+            t0 = time.perf_counter()
+            r = run.step_to(1.0, steps, record=False)
+            run.synchronize()
+            assert r["steps"] == steps
+            return time.perf_counter() - t0
+
+        def adjoint():
+            run.ch = 0.0
+            U0t = to_planes(U0).requires_grad_(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            UN = run_steps(U0t, run, steps, K)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            res = UN[0, 2:-2, 2:-2] - target
+            loss = 0.5 * (res * res).sum()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            loss.backward()
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            return t1 - t0, t3 - t2, float(loss.detach()), U0t.grad
+
+        plain()
+        adjoint()  # warm-up of both paths
+        t_fwd = min(plain() for _ in range(3))
+        runs = [adjoint() for _ in range(3)]
+        t_ckpt = min(r[0] for r in runs)
+        t_bwd = min(r[1] for r in runs)
+        loss, grad_t = runs[-1][2], runs[-1][3]
+        grad = from_planes(grad_t)
+        pairs = probe_pairs(seed, n, n)
+        adj = np.array([grad[j, i, f] for j, i, f in pairs])
+
+        # plain central differences on the GPU with the recorded dt / c_h
+        from mhd_gpu_acc_amd.adjoint import Tape
+        run.set_state(U0)
+        run.ch = 0.0
+        with Tape(run, steps, K) as tape:
+            dts, chs = tape.forward()
+        base = to_planes(U0)
+        eps = 1e-6
+        fd = []
+        for j, i, f in pairs:
+            up, um = base.clone(), base.clone()
+            up[f, j, i] += eps
+            um[f, j, i] -= eps
+            rp = replay(run, up, dts, chs)[0, 2:-2, 2:-2] - target
+            rm = replay(run, um, dts, chs)[0, 2:-2, 2:-2] - target
+            fd.append(float(((rp - rm) * (rp + rm)).sum() * 0.5) / (2 * eps))
+        fd = np.array(fd)
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    out = {"workload": f"orszag_tang_unit {n}x{n} periodic, HLLD, vanleer, {steps} steps, checkpoint every {K}",
+           "loss": loss, "forward_ms": 1e3 * t_fwd, "forward_with_checkpoints_ms": 1e3 * t_ckpt,
+           "backward_ms": 1e3 * t_bwd, "cost_ratio": (t_ckpt + t_bwd) / t_fwd, "probes": len(pairs),
+           "gpu_central_differences_norm_rel": rel(adj, fd)}
+    golden = os.path.join(ROOT, "tests", "golden", "adjoint_ot512.json")
+    if n == 512 and steps == 200 and os.path.exists(golden):
+        ref = json.load(open(golden))
+        cpu = np.array(ref["grad"])
+        out["cpu_tangent_reference_norm_rel"] = rel(adj, cpu)
+        out["cpu_tangent_reference_max_abs"] = float(np.max(np.abs(adj - cpu)))
+        for key, vals in sorted(ref.get("frozen_fd", {}).items()):
+            out[f"cpu_frozen_central_differences_eps{key}_norm_rel"] = rel(adj, np.array(vals))
+    return out
 
 
 if __name__ == "__main__":

@@ -11,7 +11,12 @@
 //       and a is not NaN, else b;
 //   std::min/std::max: the derivative follows the selected argument;
 //   |x|: derivative sign(x), 0 at x = 0;
-//   sqrt(x): derivative 1/(2 sqrt x) for x > 0, 0 at x = 0.
+//   sqrt(x): derivative 1/(2 sqrt x) for x > 0, 0 at x = 0;
+//   a limited slope whose two one-sided differences are both below rounding level
+//   (|a| <= 2^-44 (|wc| + |wm|) and |b| <= 2^-44 (|wp| + |wc|)) is differentiated as
+//   the zero slope: their signs are noise (a uniform field recovered from conserved
+//   variables), the limiter's derivative would depend only on the noise ratio, and
+//   no finite difference can observe it.  Values are never changed.
 //
 // States are in the rotated frame of physics.hpp.  A Prim adjoint carries a
 // tenth component, b2 (|B|^2 as to_prim forms it), because flux, fast_speed and
@@ -49,6 +54,10 @@ MHD_DI double d_sqrt(double r, double rb) { return r > 0.0 ? rb / (2.0 * r) : 0.
 MHD_DI double sign0(double x) { return x < 0.0 ? -1.0 : (x > 0.0 ? 1.0 : 0.0); }
 // Whether fmin(a, b) / fmax(a, b) passes the derivative to a.
 MHD_DI bool picks_a(double m, double a) { return m == a && !__builtin_isnan(a); }
+MHD_DI bool slope_noise(double wm, double wc, double wp) {
+    constexpr double tau = 0x1p-44;
+    return fabs(wc - wm) <= tau * (fabs(wc) + fabs(wm)) && fabs(wp - wc) <= tau * (fabs(wp) + fabs(wc));
+}
 
 // w = to_prim(u)
 MHD_DI void to_prim_adj(const Cons& u, const Prim& w, const Prim& wb, const Phys& ph, Cons& ub) {
@@ -703,7 +712,8 @@ MHD_DI void reconstruct_adj(const Prim& wm, const Prim& wc, const Prim& wp, doub
         wcb.f += wrb.f;                                          \
         if (pr) sb += 0.5 * wrb.f;                               \
         double ab = 0.0, bb = 0.0;                               \
-        limit_adj<LIMITER>(wc.f - wm.f, wp.f - wc.f, sb, ab, bb); \
+        if (!slope_noise(wm.f, wc.f, wp.f))                      \
+            limit_adj<LIMITER>(wc.f - wm.f, wp.f - wc.f, sb, ab, bb); \
         wcb.f += ab - bb;                                        \
         wmb.f -= ab;                                             \
         wpb.f += bb;                                             \

@@ -80,13 +80,15 @@ State orszag_tang(double x, double y) {
 }
 
 // Orszag-Tang on [0,1]^2: rho = 25/(36 pi), p = 5/(12 pi), u = -sin 2 pi y,
-// v = sin 2 pi x, B = b_scale (-sin 2 pi y, sin 4 pi x) / sqrt(4 pi).  With
+// v = sin 2 pi x, B = b (-sin 2 pi y, sin 4 pi x) / sqrt(4 pi) with b = b_scale for the
+// batch family (`ripple`) and 1 otherwise.  With
 // `ripple`, rho *= 1 + pert_amp * sum_m a_m sin(2 pi (kx x + ky y) + phi_m) / sum |a_m|;
 // with `bump`, rho += bump_amp exp(-r^2 / (2 sigma^2)) (p held, E recomputed).
 State orszag_tang_unit(double x, double y, double gamma, const mhd_ic_params* p, bool ripple, bool bump) {
     double rho = 25.0 / (36.0 * kPi);
     const double pr = 5.0 / (12.0 * kPi);
-    const double b0 = (p ? p->b_scale : 1.0) / std::sqrt(4.0 * kPi);
+    // b_scale belongs to the batch family; the bump case keeps the standard field
+    const double b0 = (ripple && p ? p->b_scale : 1.0) / std::sqrt(4.0 * kPi);
     if (ripple && p->n_modes > 0) {
         double s = 0.0, norm = 0.0;
         for (int m = 0; m < p->n_modes; ++m) {

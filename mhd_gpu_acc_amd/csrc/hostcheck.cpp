@@ -52,10 +52,13 @@ void reverse_sweep(const double* U, const double* Lin, double* Lout, const Geom&
     const int len = XDIR ? g.tnx : g.tny;
     const int n = XDIR ? g.nx : g.ny;
     const HostLam lin{Lin, g.tnx, XDIR};
+#pragma omp parallel if ((long long)nlines * len > 65536)
+    {
     std::vector<Cons> u(len), gv(len), hib(len), lob(len);
     std::vector<Prim> w(len), mb(len), cb(len), pb(len);
     std::vector<Face> lo(len), hi(len);
     std::vector<AdjFlags> fl(len);
+#pragma omp for schedule(static)
     for (int l = 0; l < nlines; ++l) {
         auto cell = [&](int a, int& i, int& j) {
             i = XDIR ? a : NG + l;
@@ -91,6 +94,7 @@ void reverse_sweep(const double* U, const double* Lin, double* Lout, const Geom&
             cell(a, i, j);
             unrot(out, XDIR, Lout + 9 * ((size_t)j * g.tnx + i));
         }
+    }
     }
 }
 

@@ -305,3 +305,27 @@ def tan_reconstruct(Wm, dWm, Wc, dWc, Wp, dWp, dtd, direction, ch, gamma, limite
                               ptr(_c9(dWp)), C.c_double(dtd), direction, C.c_double(ch), C.c_double(gamma),
                               limiter, ptr(L), ptr(dL), ptr(R), ptr(dR))
     return L, dL, R, dR
+
+
+def frozen_fd(U: np.ndarray, probes, eps: float, case: Case, dts, chs, solver: int, limiter: int,
+              bc=None, cr: float = 0.18):
+    """Frozen-branch central-difference runs (orc_frozen_fd_run).  Returns (rho0, rho_pm):
+    the final density of the unperturbed run, shape (tny, tnx), and of U +- eps e_q for
+    each probe q = (row, column, field), shape (nprobe, 2, tny, tnx)."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    probes = [tuple(int(x) for x in q) for q in probes]
+    pr = (C.c_int * (3 * len(probes)))(*[x for q in probes for x in q])
+    dts = np.ascontiguousarray(np.asarray(dts, dtype=np.float64))
+    chs = np.ascontiguousarray(np.asarray(chs, dtype=np.float64))
+    tny, tnx = case.shape[:2]
+    rho0 = np.zeros((tny, tnx))
+    rho_pm = np.zeros((len(probes), 2, tny, tnx))
+    g = case.geom()
+    lib = orc()
+    lib.orc_frozen_fd_run.argtypes = [_dp, _ip, C.c_int, C.c_double, C.POINTER(Geom), C.c_int, _dp, _dp, C.c_double,
+                                      C.c_double, C.c_int, C.c_int, _ip, _dp, _dp]
+    rc = lib.orc_frozen_fd_run(ptr(U), pr, len(probes), eps, C.byref(g), len(dts), ptr(dts), ptr(chs), cr,
+                               case.gamma, solver, limiter, ibuf(bc or case.bc), ptr(rho0), ptr(rho_pm))
+    if rc != 0:
+        raise ValueError("at most 32 probes")
+    return rho0, rho_pm

@@ -21,6 +21,8 @@
 //   SplitMix64 -> the published test vector for seed 1234567.
 //   tangent-linear model (Dual scalar) -> central finite differences of the
 //          double instantiation.
+//   frozen-branch differences (Shadow scalar) -> the double instantiation for
+//          the unperturbed copy (bitwise) and the tangent model (to O(eps^2)).
 //
 // Field order everywhere is the reference Conserved order
 // (rho, rho*u, rho*v, rho*w, Bx, By, Bz, E, psi); primitives are
@@ -112,6 +114,14 @@ int  orc_init_state(int case_id, const orc_geom* g, const orc_ic_params* p, doub
 // directional derivative d U_final / d U_0 . dU_0.
 int orc_tangent_run(double* U, double* dU, const orc_geom* g, int nsteps, const double* dts,
                     const double* chs, double cr, double gamma, int solver, int limiter, const int* bc);
+
+// Frozen-branch central differences of nsteps steps: the copies U +- eps e_q of
+// each probe q = (row, column, field) follow every branch of the unperturbed
+// trajectory (dt / c_h given).  rho0: final density of the unperturbed run
+// (tny x tnx); rho_pm: [q][+,-][tny][tnx].  nprobe <= 32; returns 1 otherwise.
+int orc_frozen_fd_run(const double* U, const int* probes, int nprobe, double eps, const orc_geom* g, int nsteps,
+                      const double* dts, const double* chs, double cr, double gamma, int solver, int limiter,
+                      const int* bc, double* rho0, double* rho_pm);
 
 // Point tangents (value and directional derivative, same branch rules) of the
 // functions above, for the dot-product tests of the hand-written adjoint.

@@ -102,3 +102,36 @@ def test_no_device_is_an_error_not_a_fallback():
     with pytest.raises(m.MHDError) as e:
         m.Run(m.get_n_case_config("brio_wu", 1))
     assert e.value.code == 6
+
+
+def test_cpp_front_compiles_links_and_reports(tmp_path):
+    """include/mhd_hip.hpp (the head/-shaped C++ front) compiles with g++, links against
+    the library, and runs: without a GPU the run constructor throws the library's
+    no-device error (no CPU fallback); with one it steps."""
+    import subprocess
+    src = tmp_path / "front.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include "mhd_hip.hpp"
+int main() {
+    try {
+        mhd_hip::BoundaryConfig bc;
+        bc.left = bc.right = bc.bottom = bc.top = mhd_hip::BoundaryType::Periodic;
+        mhd_hip::Run run(32, 32, 0.0, 1.0, 0.0, 1.0, 5.0 / 3.0, 0.8, mhd_hip::RiemannSolver::HLLD, bc);
+        double t = 0.0;
+        for (int s = 0; s < 5; ++s) { const double dt = run.compute_dt(0.8); run.advance_second_order(dt); t += dt; }
+        std::printf("ran %.17g\n", t);
+    } catch (const std::runtime_error& e) {
+        std::printf("error: %s\n", e.what());
+    }
+    return 0;
+}
+''')
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.join(root, "mhd_gpu_acc_amd")
+    exe = tmp_path / "front"
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I" + os.path.join(root, "include"),
+                    str(src), "-o", str(exe), "-L" + libdir, "-lmhd_hip", "-Wl,-rpath," + libdir,
+                    "-Wl,-rpath,/opt/rocm/lib"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120).stdout
+    assert out.startswith("ran ") or "no HIP device" in out, out

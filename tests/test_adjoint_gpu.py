@@ -1,0 +1,222 @@
+"""Discrete adjoint on the GPU (through the C ABI):
+
+* one reverse step (mhd_adjoint_step_grids) equals the host build of the same
+  arithmetic bit for bit, on grids that span several tiles and every BC mix;
+* the checkpointed tape (forward with the c_h ratchet, segment recompute, reverse)
+  equals the host chain bit for bit, and passes the dot-product test against the
+  oracle's tangent-linear model (an independent derivation);
+* the torch.autograd.Function gives the tape's gradient and, on a state without
+  rounding-level ties, matches plain GPU central differences (dt / c_h frozen);
+* the adjoint workload (Orszag-Tang 512^2, HLLD, van Leer, 200 steps, K = 20)
+  matches the oracle's CPU reference gradient (tests/golden/adjoint_ot512.json:
+  tangent model, and frozen-branch central differences at eps = 1e-6) on the 32
+  seeded probes.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import mhd_gpu_acc_amd as m
+import oracle
+from tests import hostcheck as hc
+from tests.test_adjoint_host import perturbed_ot, small_case
+from tests.util import assert_bitwise
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "adjoint_ot512.json")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def cfg_of(case):
+    return m.CaseConfig(case.nx, case.ny, 2, case.x_min, case.x_max, case.y_min, case.y_max, case.cfl, case.t_end,
+                        case.gamma, m.BoundaryConfig(*[m.BoundaryType(b) for b in case.bc]))
+
+
+STEP_CASES = [(13, 10, (0, 0, 0, 0), 2, 1), (37, 29, (2, 1, 0, 0), 2, 0), (64, 40, (1, 1, 2, 2), 0, 1),
+              (251, 7, (0, 0, 1, 2), 2, 1), (5, 90, (2, 2, 0, 0), 0, 0), (70, 70, (0, 0, 0, 0), 2, 1)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nx,ny,bc,solver,limiter", STEP_CASES)
+def test_reverse_step_matches_host(nx, ny, bc, solver, limiter):
+    import torch
+    from mhd_gpu_acc_amd.adjoint import adjoint_step_grids, from_planes, to_planes
+    say(f"[gpu] reverse step {nx}x{ny} bc={bc} solver={solver} limiter={limiter}")
+    case = small_case(nx, ny, bc)
+    U = perturbed_ot(case, nx + 7 * ny)
+    rc, dt, ch = oracle.compute_dt(U, case, case.cfl, 0.0)
+    assert rc == 0
+    Utmp, _ = oracle.advance(U, case, dt, ch, solver, limiter)
+    lam = np.random.default_rng(nx * ny).normal(0, 1, U.shape)
+    expected = hc.adjoint_step(U, Utmp, case, dt, ch, solver, limiter, lam)
+    with m.Run(cfg_of(case), solver, limiter) as run:
+        dU, dT, dL = to_planes(U), to_planes(Utmp), to_planes(lam)
+        dB = torch.empty_like(dL)
+        torch.cuda.synchronize()
+        adjoint_step_grids(run, dU.data_ptr(), dT.data_ptr(), dt, ch, dL.data_ptr(), dB.data_ptr())
+        run.synchronize()
+        got = from_planes(dL)
+    assert_bitwise(got, expected, "reverse step")
+
+
+def _oracle_chain(case, U0, nsteps, solver, limiter):
+    states, tmps, dts, chs = [U0], [], [], []
+    ch, U = 0.0, U0
+    for _ in range(nsteps):
+        rc, dt, ch = oracle.compute_dt(U, case, case.cfl, ch)
+        assert rc == 0
+        Utmp, U = oracle.advance(U, case, dt, ch, solver, limiter)
+        states.append(U)
+        tmps.append(Utmp)
+        dts.append(dt)
+        chs.append(ch)
+    return states, tmps, np.array(dts), np.array(chs)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("solver,limiter,bc,K", [(2, 1, (0, 0, 0, 0), 5), (0, 0, (1, 2, 2, 1), 12),
+                                                 (2, 0, (2, 2, 0, 0), 1)])
+def test_tape_matches_host_chain_and_tangent(solver, limiter, bc, K):
+    from mhd_gpu_acc_amd.adjoint import Tape
+    say(f"[gpu] tape 40x36 solver={solver} limiter={limiter} bc={bc} K={K}")
+    nsteps = 12
+    case = small_case(40, 36, bc)
+    U0 = perturbed_ot(case, 31)
+    states, tmps, dts, chs = _oracle_chain(case, U0, nsteps, solver, limiter)
+    rng = np.random.default_rng(3)
+    lam = rng.normal(0, 1, U0.shape)
+    v = rng.normal(0, 1, U0.shape)
+    with m.Run(cfg_of(case), solver, limiter) as run:
+        run.set_state(U0)
+        with Tape(run, nsteps, K) as tape:
+            gdts, gchs = tape.forward()
+            assert_bitwise(gdts, dts, "dt sequence")
+            assert_bitwise(gchs, chs, "c_h sequence")
+            assert_bitwise(run.get_state(), states[-1], "final state")
+            g = tape.backward(lam)
+            g2 = tape.backward(lam)  # a second backward is the same
+    A = lam
+    for s in range(nsteps - 1, -1, -1):
+        A = hc.adjoint_step(states[s], tmps[s], case, dts[s], chs[s], solver, limiter, A)
+    assert_bitwise(g, A, "tape vs host chain")
+    assert_bitwise(g2, g, "repeated backward")
+    _, dUN = oracle.tangent_run(U0, v, case, dts, chs, solver, limiter)
+    lhs, rhs = np.sum(lam * dUN), np.sum(g * v)
+    assert abs(lhs - rhs) <= 1e-11 * (np.sum(np.abs(lam * dUN)) + np.sum(np.abs(g * v))), (lhs, rhs)
+
+
+@pytest.mark.gpu
+def test_tape_rejects_unsupported():
+    from mhd_gpu_acc_amd.adjoint import Tape
+    case = small_case(8, 8, (3, 0, 0, 0))
+    with m.Run(cfg_of(case), 2, 1) as run:
+        with pytest.raises(m.MHDError):
+            Tape(run, 4, 2)
+    with m.Run(cfg_of(small_case(8, 8, (0, 0, 0, 0))), 1, 1) as run:
+        with pytest.raises(m.MHDError):
+            Tape(run, 4, 2)
+
+
+def _loss_from_planes(UN, target):
+    r = UN[0, 2:-2, 2:-2] - target
+    return 0.5 * (r * r).sum()
+
+
+def _fd(run, U0t, dts, chs, target, probe, eps):
+    """Central difference of the loss along one (row, col, field) probe, dt / c_h frozen.
+    L(+) - L(-) is formed as sum (r+ - r-)(r+ + r-)/2 to avoid cancellation."""
+    from mhd_gpu_acc_amd.adjoint import replay
+    j, i, f = probe
+    Up, Um = U0t.clone(), U0t.clone()
+    Up[f, j, i] += eps
+    Um[f, j, i] -= eps
+    rp = replay(run, Up, dts, chs)[0, 2:-2, 2:-2] - target
+    rm = replay(run, Um, dts, chs)[0, 2:-2, 2:-2] - target
+    return float(((rp - rm) * (rp + rm)).sum() * 0.5) / (2 * eps)
+
+
+@pytest.mark.gpu
+def test_autograd_function_and_finite_differences():
+    """The Function's gradient is the tape's, and on a state without rounding-level
+    ties (Orszag-Tang with seeded noise) it matches plain GPU central differences."""
+    import torch
+    from mhd_gpu_acc_amd.adjoint import Tape, from_planes, run_steps, to_planes
+    say("[gpu] autograd noisy OT 48^2 HLLD van Leer 20 steps K=7")
+    case = small_case(48, 48, (0, 0, 0, 0))
+    U0 = perturbed_ot(case, 77)
+    nsteps, K = 20, 7
+    with m.Run(cfg_of(case), 2, 1) as run:
+        run.set_state(U0 * np.r_[1.01, 1, 1, 1, 1, 1, 1, 1.01, 1])  # any target
+        run.step_to(1.0, nsteps)
+        target = to_planes(run.get_state())[0, 2:-2, 2:-2].clone()
+        run.ch = 0.0
+        U0t = to_planes(U0).requires_grad_(True)
+        UN = run_steps(U0t, run, nsteps, K)
+        loss = _loss_from_planes(UN, target)
+        loss.backward()
+        grad = from_planes(U0t.grad)
+        assert np.all(np.isfinite(grad)) and np.any(grad != 0)
+        # the same gradient through the tape API
+        lam = np.zeros(case.shape)
+        lam[2:-2, 2:-2, 0] = from_planes(UN.detach())[2:-2, 2:-2, 0] - target.cpu().numpy()
+        run.ch = 0.0
+        run.set_state(U0)
+        with Tape(run, nsteps, K) as tape:
+            dts, chs = tape.forward()
+            g2 = tape.backward(lam)
+        assert_bitwise(grad, g2, "autograd vs tape")
+        base = to_planes(U0)
+        probes = [(20, 30, 0), (40, 11, 1), (9, 45, 2), (33, 33, 4), (45, 20, 5), (12, 12, 7), (44, 44, 8),
+                  (2, 40, 3), (30, 1, 0), (35, 49, 7)]
+        adj = np.array([grad[j, i, f] for j, i, f in probes])
+        fd = np.array([_fd(run, base, dts, chs, target, q, 1e-6) for q in probes])
+    rel = np.abs(adj - fd) / np.maximum(np.abs(fd), 1e-3 * np.max(np.abs(fd)))
+    say(f"  per-probe rel {np.array2string(rel, precision=1)}")
+    assert np.linalg.norm(adj - fd) <= 1e-6 * np.linalg.norm(fd), rel
+
+
+@pytest.mark.gpu
+def test_adjoint_workload_ot512():
+    """The adjoint workload: the gradient at the 32 seeded probes against the oracle's
+    CPU reference (tangent model; frozen-branch central differences at eps 1e-6),
+    and the whole-grid projection on a seeded direction."""
+    from mhd_gpu_acc_amd.adjoint import adjoint_workload, from_planes, probe_pairs, run_steps, to_planes
+    if not os.path.exists(GOLDEN):
+        pytest.skip("adjoint_ot512.json not generated")
+    ref = json.load(open(GOLDEN))
+    w = ref["workload"]
+    say("[gpu] adjoint workload OT 512^2 HLLD van Leer 200 steps K=20")
+    cfg, U0, Ubump = adjoint_workload(w["nx"], w["seed"])
+    assert (cfg.cfl, cfg.gamma, cfg.bc.as_list()) == (w["cfl"], w["gamma"], [0, 0, 0, 0])
+    with m.Run(cfg, 2, 1, cr=w["cr"]) as run:
+        run.set_state(Ubump)
+        r = run.step_to(1.0, w["steps"])
+        assert r["steps"] == w["steps"]
+        target = to_planes(run.get_state())[0, 2:-2, 2:-2].clone()
+        run.ch = 0.0
+        U0t = to_planes(U0).requires_grad_(True)
+        UN = run_steps(U0t, run, w["steps"], 20)
+        loss = _loss_from_planes(UN, target)
+        loss.backward()
+        grad = from_planes(U0t.grad)
+    assert float(loss) == pytest.approx(ref["loss"], rel=1e-12)
+    pairs = probe_pairs(w["seed"], w["nx"], w["ny"])
+    assert [list(q) for q in pairs] == ref["pairs"]
+    adj = np.array([grad[j, i, f] for j, i, f in pairs])
+    cpu = np.array(ref["grad"])
+    say(f"  vs CPU tangent: max abs {np.max(np.abs(adj - cpu)):.3e}, max |g| {np.max(np.abs(cpu)):.3e}")
+    assert np.allclose(adj, cpu, rtol=1e-9, atol=1e-12 * np.max(np.abs(cpu)))
+    v = np.random.default_rng(w["seed"]).standard_normal(U0.shape)
+    assert float(np.sum(grad * v)) == pytest.approx(ref["direction"]["dot"], rel=1e-9)
+    fd = np.array(ref["frozen_fd"]["1e-06"])
+    nrel = np.linalg.norm(adj - fd) / np.linalg.norm(fd)
+    say(f"  vs frozen-branch central differences (eps 1e-6): norm rel {nrel:.3e}")
+    assert nrel <= 1e-6

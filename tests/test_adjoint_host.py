@@ -271,3 +271,49 @@ def test_unsupported_configurations_rejected():
         hc.adjoint_step(U, U, case, 1e-3, 1.0, 2, 1, U)
     with pytest.raises(ValueError):
         hc.adjoint_step(U, U, small_case(6, 6, (0, 0, 0, 0)), 1e-3, 1.0, 1, 1, U)
+
+
+def test_frozen_branch_differences_match_tangent():
+    """The oracle's frozen-branch central differences: the unperturbed copy is the
+    double run bit for bit, and on the unperturbed Orszag-Tang state (uniform
+    pressure with rounding noise, slopes exactly zero along the axis each IC
+    component is constant on) they converge to the tangent model: the error
+    shrinks like eps^2 down to rounding, with no eps-independent gap, which a
+    branch or derivative convention not shared by the two would leave."""
+    case = small_case(24, 24, (0, 0, 0, 0))
+    U0 = oracle.init_state(4, case)
+    r = oracle.run_loop(U0, case, 2, 1, 6, t_end=1.0)
+    probes = [(5, 7, 0), (12, 3, 1), (20, 20, 4), (9, 15, 7), (14, 2, 8), (3, 25, 5)]
+    w = np.random.default_rng(4).normal(0, 1, U0.shape[:2])
+    tan = []
+    for j, i, f in probes:
+        e = np.zeros_like(U0)
+        e[j, i, f] = 1.0
+        _, dU = oracle.tangent_run(U0, e, case, r["dts"], r["chs"], 2, 1)
+        tan.append(np.sum(w * dU[..., 0]))
+    best = np.full(len(probes), np.inf)
+    for eps in (1e-5, 1e-6, 1e-7, 1e-8):
+        rho0, rho_pm = oracle.frozen_fd(U0, probes, eps, case, r["dts"], r["chs"], 2, 1)
+        assert np.array_equal(rho0, r["U"][..., 0])
+        for q in range(len(probes)):
+            fd = np.sum(w * (rho_pm[q, 0] - rho_pm[q, 1])) / (2 * eps)
+            best[q] = min(best[q], abs(fd - tan[q]) / abs(tan[q]))
+    assert np.all(best <= 1e-6), best
+
+
+@pytest.mark.parametrize("solver,limiter", [(2, 1), (0, 0)])
+def test_step_adjoint_dot_uniform_pressure(solver, limiter):
+    """The unperturbed Orszag-Tang state: uniform density and pressure, the latter with
+    rounding noise from the conserved variables, so the rounding-level slope rule of
+    the derivative is exercised everywhere; the adjoint and the tangent model share it."""
+    case = small_case(20, 16, (0, 0, 0, 0))
+    U = oracle.init_state(4, case)
+    rc, dt, ch = oracle.compute_dt(U, case, case.cfl, 0.0)
+    Utmp, Unew = oracle.advance(U, case, dt, ch, solver, limiter)
+    rng = np.random.default_rng(6)
+    for _ in range(3):
+        v = rng.normal(0, 1, U.shape)
+        lam = rng.normal(0, 1, U.shape)
+        _, dU1 = oracle.tangent_run(U, v, case, [dt], [ch], solver, limiter)
+        A = hc.adjoint_step(U, Utmp, case, dt, ch, solver, limiter, lam)
+        close(np.sum(lam * dU1), np.sum(A * v), np.sum(np.abs(lam * dU1)) + np.sum(np.abs(A * v)))

@@ -317,6 +317,13 @@ def test_new_initial_conditions():
                         rel_tol=1e-12)
     lib.orc_cons_to_prim(ptr(np.ascontiguousarray(U8[j, i])), 5 / 3, ptr(V))
     assert math.isclose(V[7], 5 / (12 * math.pi), rel_tol=1e-12)
+    # only rho (and E through it) differ: velocities and the standard OT field are kept
+    V4 = np.zeros(9)
+    for jj in range(0, 36, 5):
+        for ii in range(0, 36, 7):
+            lib.orc_cons_to_prim(ptr(np.ascontiguousarray(U8[jj, ii])), 5 / 3, ptr(V))
+            lib.orc_cons_to_prim(ptr(np.ascontiguousarray(U4[jj, ii])), 5 / 3, ptr(V4))
+            assert np.array_equal(V[4:7], V4[4:7]) and np.allclose(V[1:4], V4[1:4], rtol=1e-14, atol=1e-15)
 
 
 def test_uniform_state_preserved_bitwise():
