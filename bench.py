@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form measurement")
     ap.add_argument("--batch-steps", type=int, default=40)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="before the timed steps, copy BYTES between two device buffers once (a kernel "
+                         "of known traffic for calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the path shards by y-slabs but no multi-GPU "
@@ -77,6 +80,14 @@ def main():
     run.set_state(U0)
     L = m.lib()
     import ctypes as C
+
+    if args.copy_probe > 0:
+        src = torch.ones(args.copy_probe // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # warm-up (also the first steps of the parity sample)
     w = run.step_to(cfg.t_end, args.warmup)
@@ -120,6 +131,7 @@ def main():
                                f"CFL {cfg.cfl}, plain step (compute_dt + advance)",
                    "nx": n, "ny": n, "solver": args.solver, "limiter": args.limiter, "cfl": cfg.cfl},
         "hbm_frac_step": value * BYTES_PER_CELL_STEP / HBM_PEAK,
+        "copy_probe_bytes": args.copy_probe,
         "kernel_ms": kern_ms,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,

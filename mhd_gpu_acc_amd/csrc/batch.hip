@@ -88,7 +88,9 @@ __global__ __launch_bounds__(256) void batch_ysweep_kernel(const BatchRun* __res
                                                            const int4* __restrict__ tiles, int ntiles,
                                                            unsigned int* __restrict__ queue,
                                                            unsigned long long* __restrict__ acc) {
+    __shared__ double ring[27 * 256];
     const int lane = threadIdx.x & 63;
+    const YRing yr{ring, (int)threadIdx.x, (int)blockDim.x};
     for (;;) {
         // Every lane takes part (lane 0 adds 1, the others 0) and lane 0's ticket is
         // broadcast: no lane-divergent branch, so the exit test is uniform and the
@@ -107,7 +109,7 @@ __global__ __launch_bounds__(256) void batch_ysweep_kernel(const BatchRun* __res
         const int j0 = NG + tl.z;
         const int j1 = j0 + tl.w;
         const double cand = ysweep_column<SOLVER, LIMITER>(src, dst, g, ph, P.dtdy, 0.5 * P.dtdy, P.damp, R.frame,
-                                                           true, i, i < NG + g.nx, j0, j1);
+                                                           true, i, i < NG + g.nx, j0, j1, yr);
         wave_max_to(acc + tl.x, cand);
     }
 }

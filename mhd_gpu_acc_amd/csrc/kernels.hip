@@ -12,7 +12,8 @@
 //             ghost_source), in one launch.
 //   ysweep  : Utmp -> Unew interior.  One lane per column marches down a segment
 //             of rows with a register pipeline (prims of two rows, the upper
-//             interface state and the lower face flux carried).  Fuses the psi
+//             interface state and the lower face flux carried; the raw rows
+//             for the update in an LDS ring).  Fuses the psi
 //             damping (solver_cpu.cpp:263-275) and the next step's CFL candidate.
 //   cfl     : stand-alone compute_dt reduction for a state not produced by ysweep.
 #include <hip/hip_runtime.h>
@@ -65,13 +66,15 @@ __global__ __launch_bounds__(256) MHD_Y_ATTR void ysweep_kernel(const double* __
                                                                 double dtd, double hdtd, double damp, int seg,
                                                                 double* __restrict__ frame,
                                                                 unsigned long long* __restrict__ acc) {
+    __shared__ double ring[27 * 256];
     const Planes<false, const double> src(Ut, g.ps);
     const Planes<false, double> dst(Un, g.ps);
     const int i = NG + blockIdx.x * blockDim.x + threadIdx.x;
     const int j0 = NG + blockIdx.y * seg;
     const int j1 = min(j0 + seg, NG + g.ny);
     const double cand = ysweep_column<SOLVER, LIMITER>(src, dst, g, ph, dtd, hdtd, damp, frame, acc != nullptr, i,
-                                                       i < NG + g.nx, j0, j1);
+                                                       i < NG + g.nx, j0, j1,
+                                                       YRing{ring, (int)threadIdx.x, (int)blockDim.x});
     if (acc) wave_max_to(acc, cand);
 }
 

@@ -161,18 +161,38 @@ MHD_DI void wave_max_to(unsigned long long* acc, double m) {
 #ifndef MHD_Y_CARRY_PRIM
 #define MHD_Y_CARRY_PRIM 1
 #endif
+
 #if MHD_Y_WAVES_PER_EU > 0
 #define MHD_Y_ATTR __attribute__((amdgpu_waves_per_eu(MHD_Y_WAVES_PER_EU)))
 #else
 #define MHD_Y_ATTR
 #endif
 
+// The raw conserved states of the last three rows a lane loaded, kept in LDS
+// (field-major, lane-consecutive, so accesses are conflict-free): the update of row
+// r-2 reads them back instead of reloading global memory, which would expose a
+// memory latency per row and double the sweep's reads.  Each lane touches only its
+// own slots, so no barrier is needed.
+struct YRing {
+    double* p;   // 27 * n doubles of LDS
+    int t, n;    // this lane's index and the lane count of the block
+    MHD_DI void put(int slot, const Cons& u) const {
+        double* q = p + slot * 9 * n + t;
+        q[0] = u.r; q[n] = u.mn; q[2 * n] = u.mt; q[3 * n] = u.mz; q[4 * n] = u.bn;
+        q[5 * n] = u.bt; q[6 * n] = u.bz; q[7 * n] = u.e; q[8 * n] = u.psi;
+    }
+    MHD_DI Cons get(int slot) const {
+        const double* q = p + slot * 9 * n + t;
+        return Cons{q[0], q[n], q[2 * n], q[3 * n], q[4 * n], q[5 * n], q[6 * n], q[7 * n], q[8 * n]};
+    }
+};
+
 // Column i, rows [j0, j1): returns the largest CFL candidate of the cells written
 // (0 when want_cfl is false or the lane is idle).
 template <int SOLVER, int LIMITER>
 MHD_DI double ysweep_column(const Planes<false, const double>& src, const Planes<false, double>& dst, const Geom& g,
                             const Phys& ph, double dtd, double hdtd, double damp, double* __restrict__ frame,
-                            bool want_cfl, int i, bool active, int j0, int j1) {
+                            bool want_cfl, int i, bool active, int j0, int j1, const YRing& ring) {
     const int ic = active ? i : NG;  // idle lanes shadow column NG (no stores)
     const bool damping = !(damp >= 1.0);  // solver_cpu.cpp:265 skips only when factor >= 1
     double cand = 0.0;
@@ -183,8 +203,11 @@ MHD_DI double ysweep_column(const Planes<false, const double>& src, const Planes
     Prim hw;     // and its primitive
 #endif
     Cons fprev;  // face flux below the cell being updated
+    int slot = 0;  // ring slot of row r; row r-2 is in slot (slot + 1) % 3
     for (int r = j0; r <= j1 + 1; ++r) {
-        const Prim w2 = to_prim(src.load(r * g.tnx + ic), ph);
+        const Cons ur = src.load(r * g.tnx + ic);
+        ring.put(slot, ur);
+        const Prim w2 = to_prim(ur, ph);
         Face lo, hi;
         reconstruct<LIMITER>(w0, w1, w2, hdtd, ph, lo, hi);  // cell r-1
         if (r >= j0 + 1) {
@@ -196,7 +219,7 @@ MHD_DI double ysweep_column(const Planes<false, const double>& src, const Planes
             if (r >= j0 + 2) {
                 const int j = r - 2;
                 const int k = j * g.tnx + ic;
-                Cons n = update(src.load(k), dtd, f, fprev);
+                Cons n = update(ring.get(slot == 2 ? 0 : slot + 1), dtd, f, fprev);
                 if (active) {
                     if (frame && (j < 2 * NG || j >= g.ny || i < 2 * NG || i >= g.nx)) frame_put(frame, g, i, j, n.psi);
                     if (damping) n.psi *= damp;
@@ -216,6 +239,7 @@ MHD_DI double ysweep_column(const Planes<false, const double>& src, const Planes
 #endif
         w0 = w1;
         w1 = w2;
+        slot = slot == 2 ? 0 : slot + 1;
     }
     return cand;
 }
