@@ -227,3 +227,20 @@ def adjoint_workload(n: int = 512, seed: int = 20261203):
     p = ICParams()
     p.bump_amp, p.bump_sigma, p.bump_cx, p.bump_cy = bump_params(seed)
     return cfg, initial_state(CaseId.OrszagTangUnit, cfg), initial_state(CaseId.OrszagTangBump, cfg, p)
+
+
+def generic_state(U0, seed: int, amp: float = 1e-3):
+    """U0 with seeded relative noise of size amp on every conserved field (rho scaled
+    by 1 + amp u, the others offset by amp n times their field's RMS; u uniform in
+    [-1, 1), n standard normal from numpy's default_rng(seed)).  The unperturbed
+    Orszag-Tang state sits on exact ties of the scheme (slopes exactly zero along the
+    axis a field is constant on, B_n = 0 lines, equal wave speeds), where the loss is
+    not differentiable at any scale a finite difference can resolve; this state is
+    generic, so central differences are well posed there."""
+    rng = np.random.default_rng(seed)
+    U = np.array(U0, dtype=np.float64, copy=True)
+    U[..., 0] *= 1.0 + amp * rng.uniform(-1.0, 1.0, U.shape[:2])
+    for k in range(1, 9):
+        rms = float(np.sqrt(np.mean(U0[..., k] ** 2))) or 1.0
+        U[..., k] += amp * rms * rng.standard_normal(U.shape[:2])
+    return U

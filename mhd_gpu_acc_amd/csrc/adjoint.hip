@@ -2,9 +2,9 @@
 // sweep, each one launch.  Memory layout as the forward (kernels.hip): nine SoA
 // planes with ghost width 2, pitch nx+4, plane stride ps.
 //
-// A workgroup of 256 lanes holds a tile of NL lines x LT cells along the sweep
-// (x: one flattened run of interior rows, NL = 1, LT = 256; y: NL = 8 columns of
-// LT = 32 rows, so that 8 consecutive lanes read 64 contiguous bytes).  Each lane
+// A workgroup of WG lanes holds a tile of NL lines x LT cells along the sweep
+// (x: one flattened run of interior rows, NL = 1; y: NL columns of LT rows, so
+// that NL consecutive lanes read 8 NL contiguous bytes).  Each lane
 // owns one cell and computes, with four LDS exchanges between neighbours along the
 // line: the cell's primitive, its interface states, the adjoint of the face above
 // it, the adjoint of its reconstruction, and finally its own output adjoint.  The
@@ -39,11 +39,11 @@ template <bool XDIR> struct LamPlanes {
     MHD_DI Cons operator()(int i, int j) const { return p.load(j * tnx + i); }
 };
 
-template <int SOLVER, int LIMITER, bool XDIR, int NL>
-__global__ __launch_bounds__(256) void adj_sweep_kernel(const double* __restrict__ U, const double* __restrict__ Lin,
-                                                        double* __restrict__ Lout, Geom g, Phys ph, double dtd,
-                                                        double hdtd, double psi_scale, Bc4 bc, int ntiles) {
-    constexpr int WG = 256, LT = WG / NL, NOUT = LT - 6;
+template <int SOLVER, int LIMITER, bool XDIR, int NL, int WG>
+__global__ __launch_bounds__(WG) void adj_sweep_kernel(const double* __restrict__ U, const double* __restrict__ Lin,
+                                                       double* __restrict__ Lout, Geom g, Phys ph, double dtd,
+                                                       double hdtd, double psi_scale, Bc4 bc, int ntiles) {
+    constexpr int LT = WG / NL, NOUT = LT - 6;
     __shared__ Lds9<WG> sW;      // cell primitives
     __shared__ Lds9<WG> sA;      // lower interface state (cons), then the bar of W[c-1]
     __shared__ Lds9<WG, 10> sB;  // its primitive, then the bar of W[c+1]
@@ -141,6 +141,20 @@ __global__ __launch_bounds__(256) void adj_sweep_kernel(const double* __restrict
     if (inb && lineok && pos >= 3 && pos < LT - 3) Planes<XDIR, double>(Lout, g.ps).store(kc, out);
 }
 
+// Tile shapes (compile-time knobs): workgroup width and, for the y pass, columns
+// per tile.  At one wave per SIMD (the kernels hold ~410 registers) a CU runs
+// whatever workgroups fit, and small workgroups keep a barrier from idling the
+// whole CU: x one wave of 64 cells (58 written), y 4 columns x 32 rows (26 written).
+#ifndef MHD_ADJ_XWG
+#define MHD_ADJ_XWG 64
+#endif
+#ifndef MHD_ADJ_YWG
+#define MHD_ADJ_YWG 128
+#endif
+#ifndef MHD_ADJ_YNL
+#define MHD_ADJ_YNL 4
+#endif
+
 template <int S, int L>
 static hipError_t launch_adjoint_sl(const AdjArgs& a) {
     const Geom g{a.nx, a.ny, a.nx + 2 * NG, a.ny + 2 * NG, a.ps};
@@ -148,17 +162,18 @@ static hipError_t launch_adjoint_sl(const AdjArgs& a) {
     const double dtdx = a.dt / a.dx, dtdy = a.dt / a.dy;
     const bool damping = !(a.damp >= 1.0);  // solver_cpu.cpp:265
     Bc4 bc{{a.bc[0], a.bc[1], a.bc[2], a.bc[3]}};
-    constexpr int NLY = 8, NOUTY = 256 / NLY - 6, NOUTX = 256 - 6;
+    constexpr int XWG = MHD_ADJ_XWG, YWG = MHD_ADJ_YWG, NLY = MHD_ADJ_YNL;
+    constexpr int NOUTY = YWG / NLY - 6, NOUTX = XWG - 6;
     dim3 gy((unsigned)((g.nx + NLY - 1) / NLY), (unsigned)((g.tny + NOUTY - 1) / NOUTY));
-    hipLaunchKernelGGL((adj_sweep_kernel<S, L, false, NLY>), gy, dim3(256), 0, a.stream, a.Utmp, a.lam, a.lam_tmp, g,
-                       ph, dtdy, 0.5 * dtdy, damping ? a.damp : 1.0, bc, 0);
+    hipLaunchKernelGGL((adj_sweep_kernel<S, L, false, NLY, YWG>), gy, dim3(YWG), 0, a.stream, a.Utmp, a.lam,
+                       a.lam_tmp, g, ph, dtdy, 0.5 * dtdy, damping ? a.damp : 1.0, bc, 0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const long long flat = (long long)g.tnx * g.ny;
     const int ntiles = (int)((flat + NOUTX - 1) / NOUTX);
-    const int nzero = (2 * NG * g.tnx + 255) / 256;
-    hipLaunchKernelGGL((adj_sweep_kernel<S, L, true, 1>), dim3((unsigned)(ntiles + nzero)), dim3(256), 0, a.stream,
-                       a.Uold, a.lam_tmp, a.lam, g, ph, dtdx, 0.5 * dtdx, 1.0, bc, ntiles);
+    const int nzero = (2 * NG * g.tnx + XWG - 1) / XWG;
+    hipLaunchKernelGGL((adj_sweep_kernel<S, L, true, 1, XWG>), dim3((unsigned)(ntiles + nzero)), dim3(XWG), 0,
+                       a.stream, a.Uold, a.lam_tmp, a.lam, g, ph, dtdx, 0.5 * dtdx, 1.0, bc, ntiles);
     return hipGetLastError();
 }
 

@@ -115,37 +115,47 @@ MHD_DI void flux_adj(const Cons& u, const Prim& w, const Cons& fb, const Phys& p
     wb.b2 += 0.5 * ptb;
 }
 
-// c = fast_speed(w)
-MHD_DI void fast_speed_adj(const Prim& w, double cb, const Phys& ph, Prim& wb) {
-    const double a2 = ph.gamma * w.p / w.r;
-    const double b2 = w.b2 / w.r;
-    const double bn2 = w.bn * w.bn / w.r;
-    const double s = a2 + b2;
-    const double disc = s * s - 4.0 * a2 * bn2;
-    const double q = sqrt(disc > 0.0 ? disc : 0.0);
-    const double c = sqrt(0.5 * (s + q));
-    const double hb = d_sqrt(c, cb);
+// Intermediates of fast_speed(w) (physics.hpp), c bitwise equal to it.
+struct FastSpeed {
+    double a2, b2, bn2, s, disc, q, c;
+};
+MHD_DI FastSpeed fast_speed_fwd(const Prim& w, const Phys& ph) {
+    FastSpeed f;
+    f.a2 = ph.gamma * w.p / w.r;
+    f.b2 = w.b2 / w.r;
+    f.bn2 = w.bn * w.bn / w.r;
+    f.s = f.a2 + f.b2;
+    f.disc = f.s * f.s - 4.0 * f.a2 * f.bn2;
+    f.q = sqrt(f.disc > 0.0 ? f.disc : 0.0);
+    f.c = sqrt(0.5 * (f.s + f.q));
+    return f;
+}
+
+// c = fast_speed(w), given its intermediates
+MHD_DI void fast_speed_adj(const Prim& w, const FastSpeed& f, double cb, const Phys& ph, Prim& wb) {
+    const double hb = d_sqrt(f.c, cb);
     double sb = 0.5 * hb;
     const double qb = 0.5 * hb;
-    const double discb = disc > 0.0 ? d_sqrt(q, qb) : 0.0;
-    sb += 2.0 * s * discb;
-    const double a2b = sb - 4.0 * bn2 * discb;
+    const double discb = f.disc > 0.0 ? d_sqrt(f.q, qb) : 0.0;
+    sb += 2.0 * f.s * discb;
+    const double a2b = sb - 4.0 * f.bn2 * discb;
     const double b2nb = sb;
-    const double bn2b = -4.0 * a2 * discb;
+    const double bn2b = -4.0 * f.a2 * discb;
     const double inv = 1.0 / w.r;
     wb.p += ph.gamma * a2b * inv;
     wb.b2 += b2nb * inv;
     wb.bn += 2.0 * w.bn * bn2b * inv;
-    wb.r -= (a2 * a2b + b2 * b2nb + bn2 * bn2b) * inv;
+    wb.r -= (f.a2 * a2b + f.b2 * b2nb + f.bn2 * bn2b) * inv;
+}
+MHD_DI void fast_speed_adj(const Prim& w, double cb, const Phys& ph, Prim& wb) {
+    fast_speed_adj(w, fast_speed_fwd(w, ph), cb, ph, wb);
 }
 
 // Wave-speed bars of S_L = fmin(vl.un - cfl, vr.un - cfr) and S_R = fmax(vl.un + cfl,
 // vr.un + cfr) (optionally against -c_h / +c_h, as HLL bounds them) into the states.
-MHD_DI void wave_speeds_adj(const Prim& vl, const Prim& vr, const Phys& ph, bool with_ch, double slb, double srb,
-                            Prim& vlb, Prim& vrb) {
-    const double cfl = fast_speed(vl, ph);
-    const double cfr = fast_speed(vr, ph);
-    const double A = vl.un - cfl, B = vr.un - cfr, C = vl.un + cfl, D = vr.un + cfr;
+MHD_DI void wave_speeds_adj(const Prim& vl, const Prim& vr, const FastSpeed& fl, const FastSpeed& fr,
+                            const Phys& ph, bool with_ch, double slb, double srb, Prim& vlb, Prim& vrb) {
+    const double A = vl.un - fl.c, B = vr.un - fr.c, C = vl.un + fl.c, D = vr.un + fr.c;
     const double il = fmin(A, B), ir = fmax(C, D);
     if (with_ch) {
         if (!picks_a(fmin(il, -ph.ch), il)) slb = 0.0;
@@ -154,8 +164,8 @@ MHD_DI void wave_speeds_adj(const Prim& vl, const Prim& vr, const Phys& ph, bool
     double cflb = 0.0, cfrb = 0.0;
     if (picks_a(il, A)) { vlb.un += slb; cflb -= slb; } else { vrb.un += slb; cfrb -= slb; }
     if (picks_a(ir, C)) { vlb.un += srb; cflb += srb; } else { vrb.un += srb; cfrb += srb; }
-    fast_speed_adj(vl, cflb, ph, vlb);
-    fast_speed_adj(vr, cfrb, ph, vrb);
+    fast_speed_adj(vl, fl, cflb, ph, vlb);
+    fast_speed_adj(vr, fr, cfrb, ph, vrb);
 }
 
 // f = hll(ul, vl, fl, ur, vr, fr)
@@ -167,8 +177,9 @@ MHD_DI void hll_adj(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& 
         acc_scaled(frb, 0.5, fb);
         return;
     }
-    const double cfl = fast_speed(vl, ph);
-    const double cfr = fast_speed(vr, ph);
+    const FastSpeed tl = fast_speed_fwd(vl, ph);
+    const FastSpeed tr = fast_speed_fwd(vr, ph);
+    const double cfl = tl.c, cfr = tr.c;
     const double sl = fmin(fmin(vl.un - cfl, vr.un - cfr), -ph.ch);
     const double sr = fmax(fmax(vl.un + cfl, vr.un + cfr), ph.ch);
     if (sl >= 0.0) { acc(flb, fb); return; }
@@ -205,7 +216,7 @@ MHD_DI void hll_adj(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& 
     const double denb = -inv * inv * invb;
     srb += denb;
     slb -= denb;
-    wave_speeds_adj(vl, vr, ph, true, slb, srb, vlb, vrb);
+    wave_speeds_adj(vl, vr, tl, tr, ph, true, slb, srb, vlb, vrb);
 }
 
 // f.bn, f.psi overwritten by glm_overwrite(f, ul, ur): consumes those bars.
@@ -235,9 +246,10 @@ MHD_DI void side_star_adj(const Cons& u, const Prim& v, double s, double sm, dou
     const double j = s - sm;
     const double d = v.r * h * j - v.bn * v.bn;
     const double bu = v.bn * v.un + v.bt * v.ut + v.bz * v.uz;
-    // e = Num / j
-    const double numb = ob.e / j;
-    double jb = -o.e * ob.e / j;
+    // e = Num / j  (the reverse uses 1/j and 1/d: its values need not be correctly rounded)
+    const double ij = 1.0 / j;
+    const double numb = ob.e * ij;
+    double jb = -o.e * numb;
     double hb = u.e * numb;
     ub.e += h * numb;
     ptb -= v.un * numb;
@@ -270,12 +282,13 @@ MHD_DI void side_star_adj(const Cons& u, const Prim& v, double s, double sm, dou
     } else {
         const double g = sm - v.un;
         const double k = v.r * h * h - v.bn * v.bn;
+        const double id = 1.0 / d;
         double gb = 0.0, kb = 0.0;
         // ut = v.ut - P/d, P = v.bn v.bt g
         {
             const double P = v.bn * v.bt * g;
-            const double Pb = -utb / d;
-            db += (P / d) * utb / d;
+            const double Pb = -utb * id;
+            db -= P * id * Pb;
             vb.ut += utb;
             vb.bn += v.bt * g * Pb;
             vb.bt += v.bn * g * Pb;
@@ -284,8 +297,8 @@ MHD_DI void side_star_adj(const Cons& u, const Prim& v, double s, double sm, dou
         // uw = v.uz - Q/d, Q = v.bn v.bz g
         {
             const double Q = v.bn * v.bz * g;
-            const double Qb = -uwb / d;
-            db += (Q / d) * uwb / d;
+            const double Qb = -uwb * id;
+            db -= Q * id * Qb;
             vb.uz += uwb;
             vb.bn += v.bz * g * Qb;
             vb.bz += v.bn * g * Qb;
@@ -293,12 +306,12 @@ MHD_DI void side_star_adj(const Cons& u, const Prim& v, double s, double sm, dou
         }
         // bt = v.bt k / d, bw = v.bz k / d
         {
-            const double nb = btb / d;
-            db -= o.bt * btb / d;
+            const double nb = btb * id;
+            db -= o.bt * nb;
             vb.bt += k * nb;
             kb += v.bt * nb;
-            const double mb = bwb / d;
-            db -= o.bw * bwb / d;
+            const double mb = bwb * id;
+            db -= o.bw * mb;
             vb.bz += k * mb;
             kb += v.bz * mb;
         }
@@ -346,8 +359,9 @@ template <bool XDIR>
 MHD_DI int hlld_adj(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr, const Phys& ph, Cons fb,
                     Cons& ulb, Prim& vlb, Cons& urb, Prim& vrb) {
     if (!prim_physical(vl) || !prim_physical(vr)) return 0;
-    const double cfl = fast_speed(vl, ph);
-    const double cfr = fast_speed(vr, ph);
+    const FastSpeed tl = fast_speed_fwd(vl, ph);
+    const FastSpeed tr = fast_speed_fwd(vr, ph);
+    const double cfl = tl.c, cfr = tr.c;
     const double sl = fmin(vl.un - cfl, vr.un - cfr);
     const double sr = fmax(vl.un + cfl, vr.un + cfr);
     if (fabs(sr - sl) < 1.0e-14) return 0;
@@ -520,10 +534,10 @@ MHD_DI int hlld_adj(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& 
         rrsb += sb1.rs;
         vrb.bn += sb1.bn;
     }
-    // side stars
+    // side stars (in a single-star region only the interface's side carries a bar)
     double ptstarb = 0.0;
-    side_star_adj(ul, vl, sl, sm, ptl, ptstar, L, Lb, ulb, vlb, slb, smb, ptlb, ptstarb);
-    side_star_adj(ur, vr, sr, sm, ptr, ptstar, R, Rb, urb, vrb, srb, smb, ptrb, ptstarb);
+    if (!outer || left) side_star_adj(ul, vl, sl, sm, ptl, ptstar, L, Lb, ulb, vlb, slb, smb, ptlb, ptstarb);
+    if (!outer || !left) side_star_adj(ur, vr, sr, sm, ptr, ptstar, R, Rb, urb, vrb, srb, smb, ptrb, ptstarb);
     // psim = 0.5 (vl.psi + vr.psi) - half_ch (vr.bn - vl.bn)
     vlb.psi += 0.5 * sb1.psi;
     vrb.psi += 0.5 * sb1.psi;
@@ -531,16 +545,17 @@ MHD_DI int hlld_adj(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& 
     vlb.bn += ph.half_ch * sb1.psi;
     // sls = sm - |vl.bn| / sql, srs = sm + |vr.bn| / sqr
     smb += slsb + srsb;
+    const double isql = 1.0 / sql, isqr = 1.0 / sqr;  // rls, rrs > 0 here
     {
-        const double qlb = -slsb;
-        vlb.bn += sign0(vl.bn) * (qlb / sql);
-        sqlb -= ql * qlb / sql;
-        const double qrb = srsb;
-        vrb.bn += sign0(vr.bn) * (qrb / sqr);
-        sqrb -= qr * qrb / sqr;
+        const double qlb = -slsb * isql;
+        vlb.bn += sign0(vl.bn) * qlb;
+        sqlb -= ql * qlb;
+        const double qrb = srsb * isqr;
+        vrb.bn += sign0(vr.bn) * qrb;
+        sqrb -= qr * qrb;
     }
-    rlsb += d_sqrt(sql, sqlb);
-    rrsb += d_sqrt(sqr, sqrb);
+    rlsb += 0.5 * sqlb * isql;
+    rrsb += 0.5 * sqrb * isqr;
     // ptstar = ptl + vl.r a1 a2
     double a1b = 0.0, b1b = 0.0;
     ptlb += ptstarb;
@@ -554,13 +569,13 @@ MHD_DI int hlld_adj(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& 
     // rls = vl.r a1 / yl, rrs = vr.r b1 / yr
     {
         const double xlb = rlsb / yl;
-        const double ylb = -rls * rlsb / yl;
+        const double ylb = -rls * xlb;
         vlb.r += a1 * xlb;
         a1b += vl.r * xlb;
         slb += ylb;
         smb -= ylb;
         const double xrb = rrsb / yr;
-        const double yrb = -rrs * rrsb / yr;
+        const double yrb = -rrs * xrb;
         vrb.r += b1 * xrb;
         b1b += vr.r * xrb;
         srb += yrb;
@@ -568,7 +583,7 @@ MHD_DI int hlld_adj(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& 
     }
     // sm = num / den
     const double numb = smb / den;
-    const double denb = -sm * smb / den;
+    const double denb = -sm * numb;
     // num = b1 vr.r vr.un - a1 vl.r vl.un + ptl - ptr
     b1b += vr.r * vr.un * numb;
     vrb.r += b1 * vr.un * numb;
@@ -593,7 +608,7 @@ MHD_DI int hlld_adj(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& 
     vlb.b2 += 0.5 * ptlb;
     vrb.p += ptrb;
     vrb.b2 += 0.5 * ptrb;
-    wave_speeds_adj(vl, vr, ph, false, slb, srb, vlb, vrb);
+    wave_speeds_adj(vl, vr, tl, tr, ph, false, slb, srb, vlb, vrb);
     return 1;
 }
 
@@ -637,10 +652,9 @@ template <int LIMITER> MHD_DI void limit_adj(double a, double b, double sb, doub
         else { if (a < b) bb += sb; else ab += sb; }
         return;
     }
-    const double D = a + b;
-    const double v = 2.0 * a * b / D;
-    const double Nb = sb / D;
-    const double Db = -v * sb / D;
+    const double iD = 1.0 / (a + b);
+    const double Nb = sb * iD;
+    const double Db = -(2.0 * a * b) * iD * Nb;
     ab += 2.0 * b * Nb + Db;
     bb += 2.0 * a * Nb + Db;
 }

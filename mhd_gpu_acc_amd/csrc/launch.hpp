@@ -16,7 +16,7 @@ struct StepArgs {
     double damp;             // exp(-dt ch / cr), computed on the host with libm
     int solver, limiter;
     int bc[4];
-    int seg;                 // y-sweep rows per lane segment (0 = default)
+    int seg;                 // y-sweep rows per lane segment (0 = chosen from the grid size)
     int xwg;                 // x-sweep workgroup width: 64, 128 or 256 (0 = default)
     int ywg;                 // y-sweep workgroup width: 64, 128 or 256 (0 = default)
     int xgrid;               // x-sweep blocks per CU (persistent), 0 = one block per segment

@@ -2,12 +2,14 @@
 //
 // Forward: the run advances nsteps steps exactly as mhd_step_to would (compute_dt
 // with the c_h ratchet, then advance), the dt and c_h of every step are recorded,
-// and the state before steps 0, K, 2K, ... is copied to a device checkpoint.
+// and the state before steps 0, K, 2K, ... is copied to a device checkpoint.  The
+// last segment steps through the segment buffers themselves.
 // Backward: for each segment, last first, the segment is recomputed from its
 // checkpoint with the recorded dt and c_h (the same kernels, so the same bits),
-// keeping every step's input U and intermediate U*; then the reverse steps of
-// adjoint.hip run from the segment's last step to its first.  Device memory:
-// ceil(nsteps/K) checkpoints + (2K+1) segment grids + 2 adjoint grids.
+// keeping every step's input U and intermediate U* -- except the last segment on
+// the first backward after a forward, which the forward left there; then the
+// reverse steps of adjoint.hip run from the segment's last step to its first.
+// Device memory: ceil(nsteps/K) checkpoints + (2K+1) segment grids + 2 adjoint grids.
 //
 // The adjoint treats the whole update as differentiable: the reference's revert of
 // a cell whose update is non-finite (solver_cpu.cpp:386-392) cannot occur on a
@@ -31,6 +33,7 @@ struct mhd_tape {
     std::vector<double> dts, chs;
     double ch_after = 0.0;
     bool recorded = false;
+    bool last_kept = false;    // S/T hold the last segment as the forward stepped it
     std::vector<double*> ck;   // checkpoints
     std::vector<double*> S;    // segment states, K+1
     std::vector<double*> T;    // segment intermediates, K
@@ -186,23 +189,41 @@ int mhd_tape_forward(mhd_tape* t, double* dts, double* chs) {
     if (!t) return fail(MHD_ERR_INVALID, "null tape");
     mhd_run* r = t->run;
     t->recorded = false;
+    t->last_kept = false;
     t->dts.assign(t->nsteps, 0.0);
     t->chs.assign(t->nsteps, 0.0);
+    const int s_last = (t->nck - 1) * t->K;  // first step of the last segment
     for (int s = 0; s < t->nsteps; ++s) {
         if (s % t->K == 0)
             MHD_HIP(hipMemcpyAsync(t->ck[s / t->K], r->U[0], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
         double dt = 0.0;
-        if (int rc = mhd_compute_dt(r, &dt)) return rc;
-        if (!std::isfinite(dt) || dt <= 0.0) return fail(MHD_ERR_DT, "dt non-finite or <= 0");
-        if (int rc = mhd_advance(r, dt)) return rc;
+        if (s < s_last) {
+            if (int rc = mhd_compute_dt(r, &dt)) return rc;
+            if (!std::isfinite(dt) || dt <= 0.0) return fail(MHD_ERR_DT, "dt non-finite or <= 0");
+            if (int rc = mhd_advance(r, dt)) return rc;
+        } else {
+            // The last segment steps through the segment buffers the backward reverses,
+            // with the arithmetic of mhd_compute_dt / mhd_advance, so it needs no recompute.
+            const int q = s - s_last;
+            if (q == 0)
+                MHD_HIP(hipMemcpyAsync(t->S[0], r->U[0], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
+            const bool reuse = q == 0 ? r->acc_valid : mhd::fused_cfl();
+            if (int rc = mhd::dt_from_state(r, t->S[q], reuse, &dt)) return rc;
+            if (!std::isfinite(dt) || dt <= 0.0) return fail(MHD_ERR_DT, "dt non-finite or <= 0");
+            if (int rc = mhd::step_grids(r, t->S[q], t->T[q], t->S[q + 1], dt)) return rc;
+        }
         t->dts[s] = dt;
         t->chs[s] = r->ch;
         if (dts) dts[s] = dt;
         if (chs) chs[s] = r->ch;
     }
+    const int m = t->nsteps - s_last;
+    MHD_HIP(hipMemcpyAsync(r->U[0], t->S[m], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
     MHD_HIP(hipStreamSynchronize(r->stream()));
+    r->acc_valid = mhd::fused_cfl();  // acc holds the CFL maximum of S[m], now U[0]
     t->ch_after = r->ch;
     t->recorded = true;
+    t->last_kept = true;
     return MHD_OK;
 }
 
@@ -215,12 +236,16 @@ int mhd_tape_backward(mhd_tape* t, const double* lam_final, double* lam0, int la
     if (bad) return fail(MHD_ERR_INVALID, "adjoint: the initial state is not finite");
     if (int rc = upload(r, lam_final, layout, t->lam[0])) return rc;
     int rc = MHD_OK;
+    const bool kept = t->last_kept;
+    t->last_kept = t->nck == 1;  // other segments overwrite the segment buffers
     for (int k = t->nck - 1; k >= 0 && !rc; --k) {
         const int s0 = k * t->K, m = std::min(t->K, t->nsteps - s0);
-        MHD_HIP(hipMemcpyAsync(t->S[0], t->ck[k], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
-        for (int q = 0; q < m && !rc; ++q) {
-            r->ch = t->chs[s0 + q];
-            rc = mhd::step_grids(r, t->S[q], t->T[q], t->S[q + 1], t->dts[s0 + q]);
+        if (!(kept && k == t->nck - 1)) {
+            MHD_HIP(hipMemcpyAsync(t->S[0], t->ck[k], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
+            for (int q = 0; q < m && !rc; ++q) {
+                r->ch = t->chs[s0 + q];
+                rc = mhd::step_grids(r, t->S[q], t->T[q], t->S[q + 1], t->dts[s0 + q]);
+            }
         }
         if (rc) break;
         if (k == t->nck - 1) {
