@@ -317,3 +317,23 @@ def test_step_adjoint_dot_uniform_pressure(solver, limiter):
         _, dU1 = oracle.tangent_run(U, v, case, [dt], [ch], solver, limiter)
         A = hc.adjoint_step(U, Utmp, case, dt, ch, solver, limiter, lam)
         close(np.sum(lam * dU1), np.sum(A * v), np.sum(np.abs(lam * dU1)) + np.sum(np.abs(A * v)))
+
+
+def test_adjoint_workload_fixture_consistent():
+    """tests/golden/adjoint_ot512.json (CPU references of the 512^2, 200-step workload):
+    the CPU adjoint agrees with the independent tangent model at every probe and on
+    the whole-grid projection, and on the generic state the frozen-branch central
+    differences converge to the tangent (error ~ eps^2 down to rounding)."""
+    import json
+    import os
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "adjoint_ot512.json")))
+    ex, ge = ref["exact"], ref["generic"]
+    tan = np.array(ex["tangent"]["probes"])
+    cpu = np.array(ex["cpu_adjoint"]["probes"])
+    assert np.max(np.abs(cpu - tan)) <= 1e-12 * np.max(np.abs(tan))
+    assert abs(ex["cpu_adjoint"]["direction_dot"] - ex["tangent"]["direction_dot"]) <= 1e-12 * abs(
+        ex["tangent"]["direction_dot"])
+    gt = np.array(ge["tangent"]["probes"])
+    e6 = np.linalg.norm(np.array(ge["frozen_fd"]["1e-06"]) - gt) / np.linalg.norm(gt)
+    e7 = np.linalg.norm(np.array(ge["frozen_fd"]["1e-07"]) - gt) / np.linalg.norm(gt)
+    assert e6 <= 3e-6 and e7 <= 3e-7 and e7 < e6 / 5, (e6, e7)
