@@ -46,11 +46,15 @@ class Tape:
     """nsteps steps of ``run`` with a checkpoint every K steps (mhd_tape_create)."""
 
     def __init__(self, run: Run, nsteps: int, K: int = 20):
+        import weakref
         L = _bind()
         h = C.c_void_p()
         check(L.mhd_tape_create(run.handle, int(nsteps), int(K), C.byref(h)))
         self._h = h
         self.run = run
+        if not hasattr(run, "_tapes"):
+            run._tapes = weakref.WeakSet()
+        run._tapes.add(self)  # closed with the run
         self.nsteps = int(nsteps)
         self.K = int(K)
         self.dts = None
@@ -60,6 +64,7 @@ class Tape:
         if getattr(self, "_h", None):
             lib().mhd_tape_destroy(self._h)
             self._h = None
+            self.run._tapes.discard(self)
 
     def __del__(self):
         try:
@@ -133,7 +138,11 @@ def _function():
             U0 = U0.contiguous()
             torch.cuda.current_stream().synchronize()
             run.set_state_device(U0.data_ptr())
-            tape = Tape(run, nsteps, K)
+            # an idle tape of the same shape (one whose backward has run) is reused
+            pool = run.__dict__.setdefault("_idle_tapes", {})
+            tape = pool.pop((nsteps, K), None)
+            if tape is None or tape._h is None:
+                tape = Tape(run, nsteps, K)
             tape.forward()
             out = torch.empty_like(U0)
             run.get_state_device(out.data_ptr())
@@ -145,7 +154,10 @@ def _function():
             grad = grad.contiguous()
             torch.cuda.current_stream().synchronize()
             lam0 = torch.empty_like(grad)
-            ctx.tape.backward_device(grad.data_ptr(), lam0.data_ptr())
+            tape = ctx.tape
+            tape.backward_device(grad.data_ptr(), lam0.data_ptr())
+            ctx.tape = None
+            tape.run.__dict__.setdefault("_idle_tapes", {})[(tape.nsteps, tape.K)] = tape
             return lam0, None, None, None
 
     return MHDSteps

@@ -41,15 +41,19 @@ struct BatchRun {   // constant for the life of the batch
     int bc[4];
 };
 
-struct BatchStep {  // rewritten by the host before every step
-    double ch, dtdx, dtdy, damp;
+// Rewritten by the host before every step.  The step constants are computed on
+// the host (same operations as a plain launch) so that the kernels read them into
+// scalar registers instead of recomputing them in vector registers.
+struct BatchStep {
+    Phys ph;
+    double dtdx, hdtdx, dtdy, hdtdy, damp;
     int cur;        // 0 or 2
     int pad;
 };
 
 MHD_DI Geom geom_of(const BatchRun& R) { return Geom{R.nx, R.ny, R.nx + 2 * NG, R.ny + 2 * NG, R.ps}; }
 // Same operations as the host builds for a plain launch (kernels.hip launch_step_sl).
-MHD_DI Phys phys_of(double gamma, double ch) { return Phys{gamma, gamma - 1.0, ch, ch * ch, 0.5 * ch, 0.5 / ch}; }
+inline Phys phys_of(double gamma, double ch) { return Phys{gamma, gamma - 1.0, ch, ch * ch, 0.5 * ch, 0.5 / ch}; }
 
 template <int SOLVER, int LIMITER, int WG>
 __global__ __launch_bounds__(WG) void batch_xsweep_kernel(const BatchRun* __restrict__ runs,
@@ -68,12 +72,12 @@ __global__ __launch_bounds__(WG) void batch_xsweep_kernel(const BatchRun* __rest
         const BatchRun& R = runs[tl.x];
         const BatchStep& P = steps[tl.x];
         const Geom g = geom_of(R);
-        const Phys ph = phys_of(R.gamma, P.ch);
-        const Planes<true, const double> src(R.U[P.cur], g.ps);
-        const Planes<true, double> dst(R.U[1], g.ps);
+        const Phys ph = P.ph;
+        const Planes<true, GlobalT<const double>> src(as_global<const double>(R.U[P.cur]), g.ps);
+        const Planes<true, GlobalT<double>> dst(as_global(R.U[1]), g.ps);
         const int k = NG * g.tnx - 2 + tl.y * (WG - 4) + t;
         const Cons u = src.load(min(k, g.tnx * g.tny - 1));
-        xsweep_segment<SOLVER, LIMITER, WG>(u, k, t, dst, g, ph, P.dtdx, 0.5 * P.dtdx, L);
+        xsweep_segment<SOLVER, LIMITER, WG>(u, k, t, dst, g, ph, P.dtdx, P.hdtdx, L);
         lds_barrier();  // LDS is rewritten by the next tile
     }
 }
@@ -82,8 +86,10 @@ __global__ __launch_bounds__(WG) void batch_xsweep_kernel(const BatchRun* __rest
 // sorted by decreasing height and handed out by a work counter (zeroed by the x
 // sweep of the same step), so the large tiles run first and the small ones fill
 // the tail of the launch.
+// Two waves per SIMD: HLLD otherwise takes 256 VGPRs plus a few AGPRs, which
+// halves the occupancy.
 template <int SOLVER, int LIMITER>
-__global__ __launch_bounds__(256) void batch_ysweep_kernel(const BatchRun* __restrict__ runs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void batch_ysweep_kernel(const BatchRun* __restrict__ runs,
                                                            const BatchStep* __restrict__ steps,
                                                            const int4* __restrict__ tiles, int ntiles,
                                                            unsigned int* __restrict__ queue,
@@ -102,14 +108,11 @@ __global__ __launch_bounds__(256) void batch_ysweep_kernel(const BatchRun* __res
         const BatchRun& R = runs[tl.x];
         const BatchStep& P = steps[tl.x];
         const Geom g = geom_of(R);
-        const Phys ph = phys_of(R.gamma, P.ch);
-        const Planes<false, const double> src(R.U[1], g.ps);
-        const Planes<false, double> dst(R.U[2 - P.cur], g.ps);
         const int i = NG + tl.y * 64 + lane;
         const int j0 = NG + tl.z;
         const int j1 = j0 + tl.w;
-        const double cand = ysweep_column<SOLVER, LIMITER>(src, dst, g, ph, P.dtdy, 0.5 * P.dtdy, P.damp, R.frame,
-                                                           true, i, i < NG + g.nx, j0, j1, yr);
+        const double cand = ysweep_column<SOLVER, LIMITER>(R.U[1], R.U[2 - P.cur], g, P.ph, P.dtdy, P.hdtdy,
+                                                           P.damp, R.frame, true, i, i < NG + g.nx, j0, j1, yr);
         wave_max_to(acc + tl.x, cand);
     }
 }
@@ -138,12 +141,13 @@ __global__ __launch_bounds__(256) void batch_cfl_kernel(const BatchRun* __restri
     const BatchRun& R = runs[tl.x];
     const BatchStep& P = steps[tl.x];
     const Geom g = geom_of(R);
-    const Phys ph = phys_of(R.gamma, P.ch);
+    const Phys ph = P.ph;
     const int q = tl.y * 256 + threadIdx.x;
     double m = 0.0;
     if (q < g.nx * g.ny) {
         const int j = NG + q / g.nx, i = NG + q % g.nx;
-        m = cfl_candidate(Planes<true, const double>(R.U[P.cur], g.ps).load(j * g.tnx + i), ph);
+        m = cfl_candidate(Planes<true, GlobalT<const double>>(as_global<const double>(R.U[P.cur]), g.ps)
+                              .load(j * g.tnx + i), ph);
     }
     wave_max_to(acc + tl.x, m);
 }
@@ -290,7 +294,7 @@ int capture_graph(mhd_batch* b) {
 // Every run's CFL maximum of its current state with its current c_h (compute_dt's
 // reduction for states that no batched y sweep produced).
 int initial_cfl(mhd_batch* b) {
-    for (int r = 0; r < b->n; ++r) b->h_steps[r].ch = b->ch[r], b->h_steps[r].cur = b->cur;
+    for (int r = 0; r < b->n; ++r) b->h_steps[r].ph = phys_of(b->d[r].gamma, b->ch[r]), b->h_steps[r].cur = b->cur;
     B_HIP(hipMemcpyAsync(b->d_steps, b->h_steps, b->n * sizeof(BatchStep), hipMemcpyHostToDevice, b->stream));
     B_HIP(hipMemsetAsync(b->d_acc, 0, b->n * sizeof(unsigned long long), b->stream));
     hipLaunchKernelGGL(batch_cfl_kernel, dim3(b->nct), dim3(256), 0, b->stream, b->d_runs, b->d_steps, b->d_ct, b->d_acc);
@@ -314,9 +318,11 @@ int plan_step(mhd_batch* b, double* dts, int* bad_run) {
         b->ch[r] = mx;
         const double dt = b->d[r].cfl * std::min(b->dx[r], b->dy[r]) / mx;
         BatchStep& P = b->h_steps[r];
-        P.ch = mx;
+        P.ph = phys_of(b->d[r].gamma, mx);
         P.dtdx = dt / b->dx[r];
+        P.hdtdx = 0.5 * P.dtdx;
         P.dtdy = dt / b->dy[r];
+        P.hdtdy = 0.5 * P.dtdy;
         P.damp = std::exp(-dt * mx / b->d[r].cr);
         P.cur = b->cur;
         if (dts) dts[r] = dt;
@@ -536,7 +542,7 @@ int mhd_batch_tune(mhd_batch* b, int reps, char* report, size_t report_len) {
     hipEvent_t e0, e1;
     B_HIP(hipEventCreate(&e0));
     B_HIP(hipEventCreate(&e1));
-    const int xwgs[] = {128, 256}, xgrids[] = {8, 16}, ysegs[] = {32, 64, 128}, ygrids[] = {2, 4, 8};
+    const int xwgs[] = {128, 256}, xgrids[] = {8, 16}, ysegs[] = {16, 32, 64}, ygrids[] = {2, 4};
     int rc = MHD_OK;
     for (int xw : xwgs)
         for (int xg : xgrids)

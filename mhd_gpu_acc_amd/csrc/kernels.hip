@@ -67,12 +67,10 @@ __global__ __launch_bounds__(256) MHD_Y_ATTR void ysweep_kernel(const double* __
                                                                 double* __restrict__ frame,
                                                                 unsigned long long* __restrict__ acc) {
     __shared__ double ring[27 * 256];
-    const Planes<false, const double> src(Ut, g.ps);
-    const Planes<false, double> dst(Un, g.ps);
     const int i = NG + blockIdx.x * blockDim.x + threadIdx.x;
     const int j0 = NG + blockIdx.y * seg;
     const int j1 = min(j0 + seg, NG + g.ny);
-    const double cand = ysweep_column<SOLVER, LIMITER>(src, dst, g, ph, dtd, hdtd, damp, frame, acc != nullptr, i,
+    const double cand = ysweep_column<SOLVER, LIMITER>(Ut, Un, g, ph, dtd, hdtd, damp, frame, acc != nullptr, i,
                                                        i < NG + g.nx, j0, j1,
                                                        YRing{ring, (int)threadIdx.x, (int)blockDim.x});
     if (acc) wave_max_to(acc, cand);

@@ -42,6 +42,13 @@ MHD_DI Cons load_rot(const double* __restrict__ U, long long ps, long long k) {
     return Planes<XDIR, const double>(U, ps).load((int)k);
 }
 
+// Global-memory view of a device pointer: its accesses become global_*
+// instructions.  A pointer read from memory (not a kernel argument) is otherwise
+// accessed with flat_* instructions, which the compiler must order against LDS
+// traffic.
+template <class T> using GlobalT = __attribute__((address_space(1))) T;
+template <class T> MHD_DI GlobalT<T>* as_global(T* p) { return (GlobalT<T>*)p; }
+
 // Workgroup barrier that orders LDS only.  __syncthreads() fences global memory
 // too, which makes every lane wait for its in-flight global loads (the next
 // segment's prefetch) before the barrier.
@@ -93,8 +100,8 @@ template <int WG> struct XLds {
 
 // One flattened segment: lane t holds cell k of the source grid (state u).
 // Lanes 2..WG-3 on an interior cell write the updated state to dst.
-template <int SOLVER, int LIMITER, int WG>
-MHD_DI void xsweep_segment(const Cons& u, int k, int t, const Planes<true, double>& dst, const Geom& g,
+template <int SOLVER, int LIMITER, int WG, class DstPlanes>
+MHD_DI void xsweep_segment(const Cons& u, int k, int t, const DstPlanes& dst, const Geom& g,
                            const Phys& ph, double dtd, double hdtd, XLds<WG>& L) {
     const Prim w = to_prim(u, ph);
     L.sw.put(t, w);
@@ -127,11 +134,11 @@ MHD_DI void xsweep_segment(const Cons& u, int k, int t, const Planes<true, doubl
 // undamped psi, solver_cpu.cpp:435-439).  Layout: [0, 2NG*nx) bottom/top strips
 // (row r: r < NG -> jb + r, else je - 2NG + r), then [2NG*nx, +2NG*ny) left/right
 // strips (column c: c < NG -> ib + c, else ie - 2NG + c).
-MHD_DI void frame_put(double* __restrict__ fr, const Geom& g, int i, int j, double psi) {
+template <class D> MHD_DI void frame_put(D* __restrict__ fr, const Geom& g, int i, int j, double psi) {
     const int li = i - NG, lj = j - NG;
     if (lj < NG) fr[lj * g.nx + li] = psi;
     if (lj >= g.ny - NG) fr[(lj - (g.ny - 2 * NG)) * g.nx + li] = psi;
-    double* side = fr + 2 * NG * g.nx;
+    D* side = fr + 2 * NG * g.nx;
     if (li < NG) side[li * g.ny + lj] = psi;
     if (li >= g.nx - NG) side[(li - (g.nx - 2 * NG)) * g.ny + lj] = psi;
 }
@@ -189,10 +196,14 @@ struct YRing {
 
 // Column i, rows [j0, j1): returns the largest CFL candidate of the cells written
 // (0 when want_cfl is false or the lane is idle).
+// Ut / Un are the bases of the source and destination grids: the restrict
+// qualifiers let the next rows' loads run ahead of this row's stores.
 template <int SOLVER, int LIMITER>
-MHD_DI double ysweep_column(const Planes<false, const double>& src, const Planes<false, double>& dst, const Geom& g,
-                            const Phys& ph, double dtd, double hdtd, double damp, double* __restrict__ frame,
-                            bool want_cfl, int i, bool active, int j0, int j1, const YRing& ring) {
+MHD_DI double ysweep_column(const double* __restrict__ Ut, double* __restrict__ Un, const Geom& g, const Phys& ph,
+                            double dtd, double hdtd, double damp, double* __restrict__ frame, bool want_cfl, int i,
+                            bool active, int j0, int j1, const YRing& ring) {
+    const Planes<false, GlobalT<const double>> src(as_global(Ut), g.ps);
+    const Planes<false, GlobalT<double>> dst(as_global(Un), g.ps);
     const int ic = active ? i : NG;  // idle lanes shadow column NG (no stores)
     const bool damping = !(damp >= 1.0);  // solver_cpu.cpp:265 skips only when factor >= 1
     double cand = 0.0;
@@ -221,7 +232,7 @@ MHD_DI double ysweep_column(const Planes<false, const double>& src, const Planes
                 const int k = j * g.tnx + ic;
                 Cons n = update(ring.get(slot == 2 ? 0 : slot + 1), dtd, f, fprev);
                 if (active) {
-                    if (frame && (j < 2 * NG || j >= g.ny || i < 2 * NG || i >= g.nx)) frame_put(frame, g, i, j, n.psi);
+                    if (frame && (j < 2 * NG || j >= g.ny || i < 2 * NG || i >= g.nx)) frame_put(as_global(frame), g, i, j, n.psi);
                     if (damping) n.psi *= damp;
                     dst.store(k, n);
                     if (want_cfl) {

@@ -148,6 +148,8 @@ class Run:
 
     def close(self):
         if getattr(self, "_h", None):
+            for t in list(getattr(self, "_tapes", ())):  # adjoint tapes hold the run's buffers
+                t.close()
             lib().mhd_run_destroy(self._h)
             self._h = None
 
