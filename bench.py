@@ -179,6 +179,23 @@ def bench_batch(steps: int, seed: int = 20261203):
 
     b, cfgs, params, states = ot_family_batch(seed)
     cells = sum(c.nx * c.ny for c in cfgs)
+    # initial states: the C++ host initialiser plus one upload per run, against one
+    # device launch for the whole batch (the second of two calls is timed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    host_states = [m.initial_state(m.CaseId.OrszagTangFamily, c, p) for c, p in zip(cfgs, params)]
+    for r, U in enumerate(host_states):
+        b.set_state(r, U)
+    torch.cuda.synchronize()
+    t_host_init = time.perf_counter() - t0
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.init_states(m.CaseId.OrszagTangFamily, params)
+        torch.cuda.synchronize()
+        t_dev_init = time.perf_counter() - t0
+    init_same = all(h(b.get_state(r)) == h(host_states[r]) for r in range(len(cfgs)))
+    del host_states
     report = b.tune(2)
     b.step(1)  # warm-up (graph upload), then re-seed the states
     for r, U in enumerate(states):
@@ -213,7 +230,9 @@ def bench_batch(steps: int, seed: int = 20261203):
     best = report.strip().splitlines()[-1]
     return {"value": steps * cells / t_batch, "unit": "cell-updates/s", "runs": len(cfgs), "cells_per_step": cells,
             "steps": steps, "sequential_plain": steps * cells / t_plain, "speedup": t_plain / t_batch,
-            "bitwise_equal_to_plain_runs_checked": same, "runs_checked": check, "tuned": best}
+            "bitwise_equal_to_plain_runs_checked": same, "runs_checked": check, "tuned": best,
+            "init_host_and_upload_ms": 1e3 * t_host_init, "init_device_ms": 1e3 * t_dev_init,
+            "init_device_bitwise_equal_to_host_all_runs": init_same}
 
 
 def bench_adjoint(n: int = 512, steps: int = 200, K: int = 20, seed: int = 20261203):
@@ -227,7 +246,8 @@ def bench_adjoint(n: int = 512, steps: int = 200, K: int = 20, seed: int = 20261
     on the Orszag-Tang state, the oracle's tangent model and the CPU adjoint (SHA-256
     of the whole gradient); on the same state with seeded 1e-3 noise (no exact ties
     of the scheme, so differences are well posed), the tangent model and
-    frozen-branch central differences (eps 1e-6, 1e-7), plus plain GPU central
+    frozen-branch central differences (two-point at eps 1e-6, 1e-7, 2e-6 and the
+    fourth-order stencil at eps 1e-6), plus plain GPU central
     differences (eps 1e-6, dt / c_h frozen, branches free) for comparison."""
     import hashlib
 
@@ -318,8 +338,12 @@ def bench_adjoint(n: int = 512, steps: int = 200, K: int = 20, seed: int = 20261
                                       == ex["cpu_adjoint"]["sha256"])
         if ge:
             out["generic_state_cpu_tangent_norm_rel"] = rel(adj_g, np.array(ge["tangent"]["probes"]))
-            for key, vals in sorted(ge.get("frozen_fd", {}).items()):
+            ffd = ge.get("frozen_fd", {})
+            for key, vals in sorted(ffd.items()):
                 out[f"generic_state_frozen_central_differences_eps{key}_norm_rel"] = rel(adj_g, np.array(vals))
+            if "1e-06" in ffd and "2e-06" in ffd:
+                fd4 = (4.0 * np.array(ffd["1e-06"]) - np.array(ffd["2e-06"])) / 3.0
+                out["generic_state_frozen_central_differences_4th_order_eps1e-06_norm_rel"] = rel(adj_g, fd4)
     return out
 
 

@@ -33,7 +33,8 @@ enum {
                                (solver_cpu.cpp:304-306 throws here) */
     MHD_ERR_DT = 4,         /* step_to: dt non-finite or <= 0 (main_cpu.cpp:205-209) */
     MHD_ERR_DT_FLOOR = 5,   /* step_to: dt < 1e-8 t_end, blow-up stop (main_cpu.cpp:210-216) */
-    MHD_ERR_NOT_BUILT = 6   /* no HIP device or kernels unavailable */
+    MHD_ERR_NOT_BUILT = 6,  /* no HIP device or kernels unavailable */
+    MHD_ERR_IO = 7          /* file I/O (snapshot CSVs, state dumps, driver log) */
 };
 
 /* ---- enums: same numeric order as the reference -------------------------------- */
@@ -101,6 +102,14 @@ int mhd_case_config_get(int case_id, int n_scale, mhd_case_config* out);
 /* initialise_grid -- init.cpp:34-43: fills every cell, ghosts included, of a host
  * AoS array (cells x 9, reference field order) from the analytic IC. */
 int mhd_init_state_host(int case_id, const mhd_case_config* geom, const mhd_ic_params* params, double* aos);
+/* initialise_grid + upload_from_aos (init.cpp:34-43, grid_gpu.cuh:139-189) on the
+ * device: fills every cell, ghosts included, of a caller's device SoA grid (9 planes of
+ * (nx+4)(ny+4) doubles) on `stream`.  Bitwise the mhd_init_state_host state.  Only the
+ * cases whose state needs no C-library transcendental: MHD_CASE_BRIO_WU(_T),
+ * MHD_CASE_SHOCK_BUBBLE, MHD_CASE_SHOCK_CLOUD and MHD_CASE_OT_FAMILY (whose sines
+ * are a portable routine shared with the host); MHD_ERR_INVALID otherwise. */
+int mhd_init_state_device(int case_id, const mhd_case_config* geom, const mhd_ic_params* params, double* soa,
+                          void* stream);
 
 /* ---- run handle: Grid2DGPU x 3 + GpuWorkspace (grid_gpu.cuh:17-223,
  *      solver_gpu.cuh:9-35): allocates Uold/Utmp/Unew on the device. ------------ */
@@ -115,6 +124,9 @@ double* mhd_state_ptr(mhd_run* run);
  * layout 2: device SoA (9 x cells, dense). */
 int mhd_set_state(mhd_run* run, const double* src, int layout);
 int mhd_get_state(mhd_run* run, double* dst, int layout);
+/* mhd_init_state_device into the run's grids: the same state as mhd_init_state_host
+ * followed by mhd_set_state, with no host work or upload. */
+int mhd_run_init_state(mhd_run* run, int case_id, const mhd_ic_params* params);
 
 /* set_gpu_physics_ch / the host mirror phys::ch_glm -- solver_gpu.cu:536-544 */
 int mhd_set_ch(mhd_run* run, double ch);
@@ -166,6 +178,10 @@ int mhd_batch_create(const mhd_run_desc* descs, int n, mhd_batch** out);
 int mhd_batch_destroy(mhd_batch* batch);
 int mhd_batch_size(const mhd_batch* batch, int* n);
 int mhd_batch_set_state(mhd_batch* batch, int run, const double* src, int layout);
+/* Every run's initial state on the device in one launch (params: one per run, or NULL
+ * for cases without parameters): as mhd_init_state_host + mhd_batch_set_state per run,
+ * bitwise, with a few hundred bytes uploaded per run.  Cases as mhd_init_state_device. */
+int mhd_batch_init_states(mhd_batch* batch, int case_id, const mhd_ic_params* params);
 int mhd_batch_get_state(mhd_batch* batch, int run, double* dst, int layout);
 int mhd_batch_get_ch(const mhd_batch* batch, int run, double* ch);
 int mhd_batch_set_ch(mhd_batch* batch, int run, double ch);
@@ -211,6 +227,65 @@ int mhd_tape_backward(mhd_tape* tape, const double* lam_final, double* lam0, int
  * lam_tmp is scratch (receives dL/dU*). */
 int mhd_adjoint_step_grids(mhd_run* run, const double* Uold, const double* Utmp, double dt, double ch, double* lam,
                            double* lam_tmp);
+
+/* ---- run driver and output formats: scripts/cpu/main_cpu.cpp:36-276,
+ *      scripts/gpu/main_gpu.cu:198-280, src/test_cases.cpp:206-264 --------------- */
+/* CaseConfig::snapshot_times / snapshot_tags (test_cases.hpp:50-54) of a case: the
+ * shock-bubble's nine Haas & Sturtevant times (test_cases.cpp:206-232) and
+ * Orszag-Tang's pi, 2 pi (:263-264); none for the other cases.  *count gets the number;
+ * with max_count >= it, times[k] and the NUL-terminated tags[k * tag_len] are filled. */
+int mhd_case_snapshots(int case_id, int n_scale, double* times, char* tags, int tag_len, int max_count, int* count);
+/* write_all_fields (main_cpu.cpp:66-91): <dir>/<prefix>_<f>.csv for f in rho, u, v, Bx,
+ * By, Bz, E, psi, p (u = rho u / rho, p from cons_to_prim), interior cells, rows = j from
+ * the top, values as "%.8e", each written to <file>.tmp and renamed; dir is created.
+ * _host takes a host SoA grid (9 planes of (nx+4)(ny+4) doubles). */
+int mhd_write_fields_csv(mhd_run* run, const char* dir, const char* prefix);
+int mhd_write_fields_csv_host(const double* soa, int nx, int ny, double gamma, const char* dir, const char* prefix);
+/* Binary FP64 state dump (no counterpart in the reference): a 128-byte header (magic
+ * "MHDSTATE", u32 version 1, u32 header bytes, i32 nx, ny, ng, 9 fields, f64 t, i64 step,
+ * f64 gamma, f64 c_h, u32 layout 1, reserved), then the 9 SoA planes of the run's
+ * current state, ghosts included, little-endian.  _read checks the grid against the run,
+ * loads it as mhd_set_state does and restores c_h, so the run continues as it would have. */
+int mhd_write_state_bin(mhd_run* run, const char* path, double t, long long step);
+int mhd_read_state_bin(mhd_run* run, const char* path, double* t, long long* step);
+
+enum {
+    MHD_DRIVE_DONE = 0,       /* reached t_end */
+    MHD_DRIVE_BAD_DT = 1,     /* dt non-finite or <= 0 (main_cpu.cpp:205-209) */
+    MHD_DRIVE_DT_FLOOR = 2,   /* dt < 1e-8 t_end (main_cpu.cpp:210-216) */
+    MHD_DRIVE_MAX_STEPS = 3   /* max_steps taken first */
+};
+typedef struct {
+    const char* case_name;            /* file prefix <case>_<platform>_<tag> */
+    const char* platform;             /* "gpu" (main_gpu.cu:198) or "cpu"; NULL = "gpu" */
+    const char* out_dir;              /* NULL: no files (--no-out) */
+    const char* log_path;             /* progress / [TIMING] lines appended here; NULL = stdout */
+    int n_scale;                      /* the [TIMING] line's n */
+    double t_end;
+    int n_snapshots;                  /* mhd_case_snapshots */
+    const double* snapshot_times;
+    const char* const* snapshot_tags;
+    double print_interval;            /* progress line when t passes it (main_cpu.cpp:221-237); <= 0: none */
+    int max_steps;                    /* 0: no limit */
+    int verbose;                      /* write the progress, [snap], Done and [TIMING] lines */
+    double t_start;                   /* restart (mhd_read_state_bin): time and step of the state; */
+    long long step_start;             /* with step_start > 0 no t0 files, passed snapshots skipped */
+} mhd_drive_opts;
+typedef struct {
+    double t;
+    long long steps;
+    double elapsed_s;
+    double last_dt;
+    int status;                       /* MHD_DRIVE_* */
+} mhd_drive_result;
+/* The time loop of main_cpu.cpp:176-276 on the run: writes <case>_<platform>_t0, then
+ * per step dt = min(compute_dt, next snapshot or t_end - t), the two dt guards, advance,
+ * a progress line with max |B| every print_interval of t, the snapshots reached, and at
+ * the end the [TIMING] line and, without a schedule, the final state tagged
+ * t<round(100 t_end)>.  One deliberate difference: the snapshot index advances without
+ * output too, so a run with out_dir = NULL reaches t_end (the reference stops there with
+ * dt = 0).  Returns MHD_OK with result->status, or an error (compute_dt's, I/O). */
+int mhd_drive(mhd_run* run, const mhd_drive_opts* opts, mhd_drive_result* result);
 
 /* ---- measurement (no counterpart in the reference, whose only timer is the
  *      steady_clock around the loop, main_cpu.cpp:191,254-269) --------------------

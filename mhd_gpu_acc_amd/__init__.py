@@ -5,7 +5,8 @@ host-side mirror of the reference's solver interface (see ``solver``).
 """
 from ._lib import MHDError, lib
 from .solver import (BoundaryConfig, BoundaryType, CaseConfig, CaseId, ICParams, Limiter, RiemannSolver, Run,
-                     get_case_config, get_n_case_config, initial_state, parse_case_id)
+                     get_case_config, get_n_case_config, init_state_device, initial_state, parse_case_id)
 
 __all__ = ["MHDError", "lib", "BoundaryConfig", "BoundaryType", "CaseConfig", "CaseId", "ICParams", "Limiter",
-           "RiemannSolver", "Run", "get_case_config", "get_n_case_config", "initial_state", "parse_case_id"]
+           "RiemannSolver", "Run", "get_case_config", "get_n_case_config", "init_state_device", "initial_state",
+           "parse_case_id"]

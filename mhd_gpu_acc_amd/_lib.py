@@ -83,6 +83,8 @@ def lib():
     L.mhd_sync.argtypes = [C.c_void_p]
     L.mhd_case_config_get.argtypes = [C.c_int, C.c_int, C.POINTER(CaseCfg)]
     L.mhd_init_state_host.argtypes = [C.c_int, C.POINTER(CaseCfg), C.POINTER(ICParams), C.c_void_p]
+    L.mhd_init_state_device.argtypes = [C.c_int, C.POINTER(CaseCfg), C.POINTER(ICParams), C.c_void_p, C.c_void_p]
+    L.mhd_run_init_state.argtypes = [C.c_void_p, C.c_int, C.POINTER(ICParams)]
     L.mhd_device_count.argtypes = [_ip]
     L.mhd_profile.argtypes = [C.c_void_p, C.c_int]
     L.mhd_profile_read.argtypes = [C.c_void_p, _dp, _ip]

@@ -256,3 +256,44 @@ def generic_state(U0, seed: int, amp: float = 1e-3):
         rms = float(np.sqrt(np.mean(U0[..., k] ** 2))) or 1.0
         U[..., k] += amp * rms * rng.standard_normal(U.shape[:2])
     return U
+
+
+def main(argv=None):
+    """One tape forward and backward of the adjoint workload (HLLD, van Leer) on the
+    GPU, with wall times; a small driver for profilers:
+
+        python -m mhd_gpu_acc_amd.adjoint --n 512 --steps 4 --K 2
+    """
+    import argparse
+    import json
+    import time
+
+    import torch
+
+    from .solver import Limiter, RiemannSolver
+
+    ap = argparse.ArgumentParser(description=main.__doc__.splitlines()[0])
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--K", type=int, default=20)
+    a = ap.parse_args(argv)
+    cfg, U0, _ = adjoint_workload(a.n)
+    lam = to_planes(np.random.default_rng(1).standard_normal(U0.shape))
+    out = torch.empty_like(lam)
+    with Run(cfg, RiemannSolver.HLLD, Limiter.VAN_LEER) as run:
+        run.set_state(U0)
+        run.ch = 0.0
+        with Tape(run, a.steps, a.K) as tape:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            tape.forward()
+            t1 = time.perf_counter()
+            tape.backward_device(lam.data_ptr(), out.data_ptr())
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+    print(json.dumps(dict(n=a.n, steps=a.steps, K=a.K, forward_ms=1e3 * (t1 - t0), backward_ms=1e3 * (t2 - t1),
+                          grad_abs_sum=float(out.abs().sum()))), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -13,7 +13,7 @@ import numpy as np
 
 from ._lib import ICParams, RunDesc, check, lib
 from .solver import BoundaryConfig, BoundaryType, CaseConfig, CaseId, Limiter, RiemannSolver, get_case_config, \
-    initial_state
+    initial_state, parse_case_id
 
 __all__ = ["Batch", "plan_ot_family", "ot_family_batch"]
 
@@ -25,6 +25,7 @@ def _bind():
     L.mhd_batch_create.argtypes = [C.POINTER(RunDesc), C.c_int, C.POINTER(C.c_void_p)]
     L.mhd_batch_destroy.argtypes = [C.c_void_p]
     L.mhd_batch_set_state.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    L.mhd_batch_init_states.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
     L.mhd_batch_get_state.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
     L.mhd_batch_get_ch.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
     L.mhd_batch_set_ch.argtypes = [C.c_void_p, C.c_int, C.c_double]
@@ -74,6 +75,17 @@ class Batch:
             raise ValueError(f"state shape {U.shape} != {self.cfgs[r].shape}")
         check(lib().mhd_batch_set_state(self._h, r, U.ctypes.data_as(C.c_void_p), 0))
 
+    def init_states(self, case, params=None):
+        """Every run's analytic initial state on the device in one launch
+        (mhd_batch_init_states): bitwise set_state(r, initial_state(case, cfgs[r],
+        params[r])) for every run.  ``params``: one ICParams per run, or None."""
+        arr = None
+        if params is not None:
+            if len(params) != len(self.cfgs):
+                raise ValueError(f"{len(params)} parameter sets for {len(self.cfgs)} runs")
+            arr = (ICParams * len(params))(*params)
+        check(lib().mhd_batch_init_states(self._h, int(parse_case_id(case)), arr))
+
     def get_state(self, r: int) -> np.ndarray:
         U = np.empty(self.cfgs[r].shape)
         check(lib().mhd_batch_get_state(self._h, r, U.ctypes.data_as(C.c_void_p), 0))
@@ -114,9 +126,12 @@ def plan_ot_family(seed: int, n: int = 64):
     return list(sizes), [params[r] for r in range(n)]
 
 
-def ot_family_batch(seed: int, n: int = 64, solver=RiemannSolver.HLLD, limiter=Limiter.VAN_LEER, scale: int = 1):
+def ot_family_batch(seed: int, n: int = 64, solver=RiemannSolver.HLLD, limiter=Limiter.VAN_LEER, scale: int = 1,
+                    device_init: bool = False):
     """The 64-run batch: Orszag-Tang family ICs on [0,1]^2, periodic, CFL 0.8.
-    ``scale`` divides the grid sizes (for small test batches).  Returns (batch, cfgs, params, states)."""
+    ``scale`` divides the grid sizes (for small test batches).  With ``device_init`` the
+    states are computed on the device (Batch.init_states) and ``states`` is None.
+    Returns (batch, cfgs, params, states)."""
     sizes, params = plan_ot_family(seed, n)
     base = get_case_config(CaseId.OrszagTangFamily)
     cfgs, states = [], []
@@ -124,8 +139,12 @@ def ot_family_batch(seed: int, n: int = 64, solver=RiemannSolver.HLLD, limiter=L
         c = CaseConfig(s // scale, s // scale, 2, base.x_min, base.x_max, base.y_min, base.y_max, base.cfl, base.t_end,
                        p.gamma, BoundaryConfig(*[BoundaryType.Periodic] * 4))
         cfgs.append(c)
-        states.append(initial_state(CaseId.OrszagTangFamily, c, p))
+        if not device_init:
+            states.append(initial_state(CaseId.OrszagTangFamily, c, p))
     b = Batch(cfgs, solver, limiter)
+    if device_init:
+        b.init_states(CaseId.OrszagTangFamily, params)
+        return b, cfgs, params, None
     for r, U in enumerate(states):
         b.set_state(r, U)
     return b, cfgs, params, states

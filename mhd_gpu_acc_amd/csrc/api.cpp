@@ -226,6 +226,52 @@ int mhd_set_state(mhd_run* r, const double* src, int layout) {
     return MHD_OK;
 }
 
+int mhd_run_init_state(mhd_run* r, int case_id, const mhd_ic_params* p) {
+    if (!r) return fail(MHD_ERR_INVALID, "null run");
+    if (!mhd::ic_on_device(case_id))
+        return fail(MHD_ERR_INVALID, "this case has no device initial state (its host state calls the C library's "
+                                     "sin/tanh/exp): use mhd_init_state_host and mhd_set_state");
+    if (case_id == MHD_CASE_OT_FAMILY && !p) return fail(MHD_ERR_INVALID, "the OT family needs its parameters");
+    mhd::IcRun R{};
+    for (int g = 0; g < 3; ++g) R.U[g] = r->U[g];  // Utmp/Unew start as copies (main_cpu.cpp:173-174)
+    R.ngrids = 3;
+    R.case_id = case_id;
+    R.tnx = r->tnx;
+    R.tny = r->tny;
+    R.ps = (long long)r->cells;
+    R.x_min = r->d.x_min;
+    R.y_min = r->d.y_min;
+    R.dx = (r->d.x_max - r->d.x_min) / static_cast<double>(r->d.nx);  // as the host initialiser
+    R.dy = (r->d.y_max - r->d.y_min) / static_cast<double>(r->d.ny);
+    if (p) R.p = *p;
+    MHD_HIP(mhd::launch_ic_one(R, r->stream()));
+    MHD_HIP(hipStreamSynchronize(r->stream()));
+    r->acc_valid = false;
+    return MHD_OK;
+}
+
+int mhd_init_state_device(int case_id, const mhd_case_config* gm, const mhd_ic_params* p, double* soa, void* stream) {
+    if (!gm || !soa || gm->nx < 1 || gm->ny < 1 || gm->ng != 2) return fail(MHD_ERR_INVALID, "bad geometry");
+    if (!mhd::ic_on_device(case_id))
+        return fail(MHD_ERR_INVALID, "this case has no device initial state (its host state calls the C library's "
+                                     "sin/tanh/exp): use mhd_init_state_host");
+    if (case_id == MHD_CASE_OT_FAMILY && !p) return fail(MHD_ERR_INVALID, "the OT family needs its parameters");
+    mhd::IcRun R{};
+    R.U[0] = soa;
+    R.ngrids = 1;
+    R.case_id = case_id;
+    R.tnx = gm->nx + 4;
+    R.tny = gm->ny + 4;
+    R.ps = (long long)R.tnx * R.tny;
+    R.x_min = gm->x_min;
+    R.y_min = gm->y_min;
+    R.dx = (gm->x_max - gm->x_min) / static_cast<double>(gm->nx);
+    R.dy = (gm->y_max - gm->y_min) / static_cast<double>(gm->ny);
+    if (p) R.p = *p;
+    MHD_HIP(mhd::launch_ic_one(R, static_cast<hipStream_t>(stream)));
+    return MHD_OK;
+}
+
 int mhd_get_state(mhd_run* r, double* dst, int layout) {
     if (!r || !dst) return fail(MHD_ERR_INVALID, "null argument");
     const size_t n = r->cells;

@@ -459,6 +459,46 @@ int mhd_batch_set_state(mhd_batch* b, int r, const double* src, int layout) {
     return MHD_OK;
 }
 
+int mhd_batch_init_states(mhd_batch* b, int case_id, const mhd_ic_params* params) {
+    if (!b) return set_error(MHD_ERR_INVALID, "null batch");
+    if (!ic_on_device(case_id))
+        return set_error(MHD_ERR_INVALID, "this case has no device initial state (its host state calls the C "
+                                          "library's sin/tanh/exp): use mhd_init_state_host and mhd_batch_set_state");
+    if (case_id == MHD_CASE_OT_FAMILY && !params) return set_error(MHD_ERR_INVALID, "the OT family needs parameters");
+    std::vector<IcRun> ic(b->n);
+    long long total = 0;
+    for (int r = 0; r < b->n; ++r) {
+        const BatchRun& R = b->runs[r];
+        const mhd_run_desc& d = b->d[r];
+        IcRun& q = ic[r];
+        q = IcRun{};
+        q.U[0] = R.U[b->cur];  // the three grids start equal, as after mhd_batch_set_state
+        q.U[1] = R.U[1];
+        q.U[2] = R.U[2 - b->cur];
+        q.ngrids = 3;
+        q.case_id = case_id;
+        q.tnx = R.nx + 2 * NG;
+        q.tny = R.ny + 2 * NG;
+        q.ps = R.ps;
+        q.first = total;
+        q.x_min = d.x_min;
+        q.y_min = d.y_min;
+        q.dx = (d.x_max - d.x_min) / static_cast<double>(d.nx);  // as the host initialiser
+        q.dy = (d.y_max - d.y_min) / static_cast<double>(d.ny);
+        if (params) q.p = params[r];
+        total += R.ps;
+    }
+    IcRun* d_ic = nullptr;
+    B_HIP(hipMalloc(&d_ic, ic.size() * sizeof(IcRun)));
+    hipError_t e = hipMemcpyAsync(d_ic, ic.data(), ic.size() * sizeof(IcRun), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = launch_ic_batch(d_ic, b->n, total, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(d_ic);
+    B_HIP(e);
+    b->acc_valid = false;
+    return MHD_OK;
+}
+
 int mhd_batch_get_state(mhd_batch* b, int r, double* dst, int layout) {
     if (int rc = check_run(b, r)) return rc;
     if (!dst) return set_error(MHD_ERR_INVALID, "null state");

@@ -3,46 +3,24 @@
 // plus the cases this build adds (unit-box Orszag-Tang, MHD shock-cloud,
 // transmissive Brio-Wu, the batch Orszag-Tang family and the Gaussian bump).
 //
-// The initial states are evaluated on the host with the C library's sin, tanh,
+// The reference's cases are evaluated on the host with the C library's sin, tanh,
 // exp and sqrt, as the reference does, so a state built here is bitwise the
-// state the reference (or any CPU code calling the same libm) builds.
+// state the reference (or any CPU code calling the same libm) builds.  The cases
+// in ic.hpp (Brio-Wu, shock-bubble, shock-cloud, the batch family) are the same
+// source the device initialiser runs.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 
 #include "../../include/mhd_hip.h"
+#include "ic.hpp"
 
 namespace {
 
-constexpr double kPi = 3.14159265358979323846;  // M_PI (test_cases.cpp:15-17)
-
-struct State {
-    double q[9];
-};
-
-// make_mhd (test_cases.cpp:24-35): conserved state from primitives, psi = 0.
-State from_primitives(double rho, double u, double v, double w, double bx, double by, double bz, double p,
-                      double gamma) {
-    const double b2 = bx * bx + by * by + bz * bz;
-    const double ke = 0.5 * rho * (u * u + v * v + w * w);
-    const double e = p / (gamma - 1.0) + ke + 0.5 * b2;
-    return State{{rho, rho * u, rho * v, rho * w, bx, by, bz, e, 0.0}};
-}
-
-// Counter-based SplitMix64 (Steele, Lea & Flood 2014): one mixing step of the
-// state seed ^ (j << 32 | i); the top 53 bits give a uniform double in [0, 1).
-uint64_t splitmix_next(uint64_t& s) {
-    s += 0x9E3779B97F4A7C15ull;
-    uint64_t z = s;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-double uniform_at(uint64_t seed, int i, int j) {
-    uint64_t s = seed ^ ((static_cast<uint64_t>(static_cast<uint32_t>(j)) << 32) |
-                         static_cast<uint64_t>(static_cast<uint32_t>(i)));
-    return static_cast<double>(splitmix_next(s) >> 11) * 0x1.0p-53;
-}
+using mhd_ic::kPi;
+using mhd_ic::State;
+using mhd_ic::from_primitives;
+using mhd_ic::splitmix_next;
 
 // Kelvin-Helmholtz (test_cases.cpp:43-63)
 State kelvin_helmholtz(double x, double y) {
@@ -50,26 +28,6 @@ State kelvin_helmholtz(double x, double y) {
     const double uy = 0.25 * std::sin(2.0 * kPi * x) *
                       (std::exp(-100.0 * (y + 0.5) * (y + 0.5)) - std::exp(-100.0 * (y - 0.5) * (y - 0.5)));
     return from_primitives(1.0, ux, uy, 0.0, 1.0, 0.0, 0.0, 50.0, 1.4);
-}
-
-// Mach 1.22 shock-bubble, air/helium (test_cases.cpp:76-106)
-State shock_bubble(double x, double y) {
-    const double gamma = 1.4, rho_air = 1.29, rho_he = 0.214, p0 = 1.01325e5, ms = 1.22;
-    const double c1 = std::sqrt(gamma * p0 / rho_air);
-    const double vs = ms * c1;
-    const double rho2 = rho_air * (gamma + 1.0) * ms * ms / ((gamma - 1.0) * ms * ms + 2.0);
-    const double p2 = p0 * (2.0 * gamma * ms * ms - (gamma - 1.0)) / (gamma + 1.0);
-    const double u2 = vs * (1.0 - rho_air / rho2);
-    if (x < 0.005) return from_primitives(rho2, u2, 0.0, 0.0, 0.0, 0.0, 0.0, p2, gamma);
-    const double r2 = (x - 0.035) * (x - 0.035) + (y - 0.0445) * (y - 0.0445);
-    if (r2 < 0.025 * 0.025) return from_primitives(rho_he, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, p0, gamma);
-    return from_primitives(rho_air, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, p0, gamma);
-}
-
-// Brio-Wu (test_cases.cpp:117-125)
-State brio_wu(double x) {
-    if (x < 0.5) return from_primitives(1.0, 0.0, 0.0, 0.0, 0.75, 1.0, 0.0, 1.0, 2.0);
-    return from_primitives(0.125, 0.0, 0.0, 0.0, 0.75, -1.0, 0.0, 0.1, 2.0);
 }
 
 // Orszag-Tang on [0, 2 pi]^2 (test_cases.cpp:139-148)
@@ -80,44 +38,19 @@ State orszag_tang(double x, double y) {
 }
 
 // Orszag-Tang on [0,1]^2: rho = 25/(36 pi), p = 5/(12 pi), u = -sin 2 pi y,
-// v = sin 2 pi x, B = b (-sin 2 pi y, sin 4 pi x) / sqrt(4 pi) with b = b_scale for the
-// batch family (`ripple`) and 1 otherwise.  With
-// `ripple`, rho *= 1 + pert_amp * sum_m a_m sin(2 pi (kx x + ky y) + phi_m) / sum |a_m|;
-// with `bump`, rho += bump_amp exp(-r^2 / (2 sigma^2)) (p held, E recomputed).
-State orszag_tang_unit(double x, double y, double gamma, const mhd_ic_params* p, bool ripple, bool bump) {
+// v = sin 2 pi x, B = (-sin 2 pi y, sin 4 pi x) / sqrt(4 pi); with `bump`,
+// rho += bump_amp exp(-r^2 / (2 sigma^2)) (p held, E recomputed).
+State orszag_tang_unit(double x, double y, const mhd_ic_params* p, bool bump) {
     double rho = 25.0 / (36.0 * kPi);
     const double pr = 5.0 / (12.0 * kPi);
-    // b_scale belongs to the batch family; the bump case keeps the standard field
-    const double b0 = (ripple && p ? p->b_scale : 1.0) / std::sqrt(4.0 * kPi);
-    if (ripple && p->n_modes > 0) {
-        double s = 0.0, norm = 0.0;
-        for (int m = 0; m < p->n_modes; ++m) {
-            s += p->mode_amp[m] * std::sin(2.0 * kPi * (p->mode_kx[m] * x + p->mode_ky[m] * y) + p->mode_phase[m]);
-            norm += std::fabs(p->mode_amp[m]);
-        }
-        rho = rho * (1.0 + p->pert_amp * (s / norm));
-    }
+    const double b0 = 1.0 / std::sqrt(4.0 * kPi);
     if (bump) {
         const double r2 = (x - p->bump_cx) * (x - p->bump_cx) + (y - p->bump_cy) * (y - p->bump_cy);
         rho = rho + p->bump_amp * std::exp(-r2 / (2.0 * p->bump_sigma * p->bump_sigma));
     }
     const double sy = std::sin(2.0 * kPi * y);
     const double sx = std::sin(2.0 * kPi * x);
-    return from_primitives(rho, -sy, sx, 0.0, -sy * b0, std::sin(4.0 * kPi * x) * b0, 0.0, pr, gamma);
-}
-
-// MHD shock-cloud on [0,2]x[0,1]: post-shock slab x < 0.6 moving right into an
-// ambient state; a dense bubble r < 0.15 at (0.8, 0.5) in pressure balance with
-// +-1 % seeded density noise.
-State shock_cloud(double x, double y, int i, int j, uint64_t seed) {
-    const double gamma = 5.0 / 3.0;
-    if (x < 0.6) return from_primitives(3.86859, 11.2536, 0.0, 0.0, 0.0, 2.1826, -2.1826, 167.345, gamma);
-    const double r2 = (x - 0.8) * (x - 0.8) + (y - 0.5) * (y - 0.5);
-    if (r2 < 0.15 * 0.15) {
-        const double rho = 10.0 * (1.0 + 0.01 * (2.0 * uniform_at(seed, i, j) - 1.0));
-        return from_primitives(rho, 0.0, 0.0, 0.0, 0.0, 0.56418, 0.56418, 1.0, gamma);
-    }
-    return from_primitives(1.0, 0.0, 0.0, 0.0, 0.0, 0.56418, 0.56418, 1.0, gamma);
+    return from_primitives(rho, -sy, sx, 0.0, -sy * b0, std::sin(4.0 * kPi * x) * b0, 0.0, pr, 5.0 / 3.0);
 }
 
 }  // namespace
@@ -161,7 +94,6 @@ int mhd_init_state_host(int case_id, const mhd_case_config* gm, const mhd_ic_par
     const int tnx = gm->nx + 2 * gm->ng, tny = gm->ny + 2 * gm->ng;
     const double dx = (gm->x_max - gm->x_min) / static_cast<double>(gm->nx);
     const double dy = (gm->y_max - gm->y_min) / static_cast<double>(gm->ny);
-    const double gamma_family = p ? p->gamma : 5.0 / 3.0;
     for (int j = 0; j < tny; ++j) {
         const double y = gm->y_min + (static_cast<double>(j - gm->ng) + 0.5) * dy;  // init.cpp:13-15
         for (int i = 0; i < tnx; ++i) {
@@ -169,14 +101,14 @@ int mhd_init_state_host(int case_id, const mhd_case_config* gm, const mhd_ic_par
             State s;
             switch (case_id) {
                 case MHD_CASE_KELVIN_HELMHOLTZ: s = kelvin_helmholtz(x, y); break;
-                case MHD_CASE_SHOCK_BUBBLE: s = shock_bubble(x, y); break;
+                case MHD_CASE_SHOCK_BUBBLE: s = mhd_ic::shock_bubble(x, y); break;
                 case MHD_CASE_BRIO_WU:
-                case MHD_CASE_BRIO_WU_T: s = brio_wu(x); break;
+                case MHD_CASE_BRIO_WU_T: s = mhd_ic::brio_wu(x); break;
                 case MHD_CASE_ORSZAG_TANG: s = orszag_tang(x, y); break;
-                case MHD_CASE_ORSZAG_TANG_UNIT: s = orszag_tang_unit(x, y, 5.0 / 3.0, nullptr, false, false); break;
-                case MHD_CASE_SHOCK_CLOUD: s = shock_cloud(x, y, i, j, p ? p->seed : 0); break;
-                case MHD_CASE_OT_FAMILY: s = orszag_tang_unit(x, y, gamma_family, p, true, false); break;
-                case MHD_CASE_OT_BUMP: s = orszag_tang_unit(x, y, 5.0 / 3.0, p, false, true); break;
+                case MHD_CASE_ORSZAG_TANG_UNIT: s = orszag_tang_unit(x, y, nullptr, false); break;
+                case MHD_CASE_SHOCK_CLOUD: s = mhd_ic::shock_cloud(x, y, i, j, p ? p->seed : 0); break;
+                case MHD_CASE_OT_FAMILY: s = mhd_ic::ot_family(x, y, *p); break;
+                case MHD_CASE_OT_BUMP: s = orszag_tang_unit(x, y, p, true); break;
                 default: return MHD_ERR_INVALID;
             }
             std::memcpy(aos + 9 * (static_cast<size_t>(j) * tnx + i), s.q, sizeof s.q);

@@ -99,6 +99,20 @@ __global__ __launch_bounds__(256) void cfl_kernel(const double* __restrict__ U, 
     wave_max_to(acc, m);
 }
 
+// max over interior cells of |B| = sqrt(Bx^2 + By^2 + Bz^2), the driver's progress
+// column (main_cpu.cpp:224-230); std::max order, so NaN cells are skipped as there.
+__global__ __launch_bounds__(256) void max_b_kernel(const double* __restrict__ U, Geom g,
+                                                    unsigned long long* __restrict__ acc) {
+    const long long n = (long long)g.nx * g.ny;
+    double m = 0.0;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
+        const long long k = (long long)(NG + q / g.nx) * g.tnx + NG + q % g.nx;
+        const double bx = U[F_BX * g.ps + k], by = U[F_BY * g.ps + k], bz = U[F_BZ * g.ps + k];
+        m = std_max(m, __builtin_sqrt(bx * bx + by * by + bz * bz));
+    }
+    wave_max_to(acc, m);
+}
+
 }  // namespace mhd
 
 // ===========================================================================================
@@ -184,6 +198,14 @@ hipError_t launch_cfl(const double* U, int nx, int ny, long long ps, double gamm
     long long nb = (n + 255) / 256;
     if (nb > 4096) nb = 4096;
     hipLaunchKernelGGL(cfl_kernel, dim3((unsigned)nb), dim3(256), 0, s, U, g, ph, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_max_b(const double* U, int nx, int ny, long long ps, unsigned long long* acc, hipStream_t s) {
+    const Geom g{nx, ny, nx + 2 * NG, ny + 2 * NG, ps};
+    long long nb = ((long long)nx * ny + 255) / 256;
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(max_b_kernel, dim3((unsigned)nb), dim3(256), 0, s, U, g, acc);
     return hipGetLastError();
 }
 

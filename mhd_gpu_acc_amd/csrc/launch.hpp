@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/mhd_hip.h"
+
 namespace mhd {
 
 struct StepArgs {
@@ -55,5 +57,25 @@ bool fused_cfl();
 hipError_t launch_cfl(const double* U, int nx, int ny, long long ps, double gamma, double ch,
                       unsigned long long* acc, hipStream_t s);
 hipError_t launch_bc(double* U, const double* prev, int nx, int ny, long long ps, const int* bc, hipStream_t s);
+// *acc (zeroed by the caller) gets the bits of max |B| over the interior cells of U.
+hipError_t launch_max_b(const double* U, int nx, int ny, long long ps, unsigned long long* acc, hipStream_t s);
+
+// A grid set to initialise on the device (init.hip): the first ngrids of U get
+// the same state.  Cells are numbered row-major over the tnx x tny grid, ghosts
+// included; in a batch launch this run owns [first, first + ps).
+struct IcRun {
+    double* U[3];
+    int ngrids;
+    int case_id;
+    int tnx, tny;
+    long long ps;
+    long long first;
+    double x_min, y_min, dx, dy;
+    mhd_ic_params p;
+};
+// Whether case_id has a device initial state (the cases of ic.hpp).
+bool ic_on_device(int case_id);
+hipError_t launch_ic_one(const IcRun& R, hipStream_t s);
+hipError_t launch_ic_batch(const IcRun* d_runs, int nruns, long long total, hipStream_t s);
 
 }  // namespace mhd

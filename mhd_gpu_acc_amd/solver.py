@@ -118,6 +118,16 @@ def get_case_config(case) -> CaseConfig:
     return get_n_case_config(case, 1)
 
 
+def init_state_device(case, cfg: CaseConfig, params: ICParams | None, ptr: int, stream: int = 0):
+    """The initial state computed on the device into a caller's dense SoA grid at device
+    address ``ptr`` (9 planes of (nx+4)(ny+4) doubles), asynchronously on ``stream``:
+    bitwise ``initial_state`` for the cases that have a device state."""
+    cc = cfg._c()
+    check(lib().mhd_init_state_device(int(parse_case_id(case)), C.byref(cc),
+                                      C.byref(params) if params is not None else None, C.c_void_p(ptr),
+                                      C.c_void_p(stream) if stream else None))
+
+
 def initial_state(case, cfg: CaseConfig, params: ICParams | None = None) -> np.ndarray:
     """initialise_grid (init.cpp:34-43): every cell, ghosts included, from the analytic IC."""
     U = np.zeros(cfg.shape)
@@ -188,6 +198,13 @@ class Run:
         U = np.empty(self.cfg.shape)
         check(lib().mhd_get_state(self._h, U.ctypes.data_as(C.c_void_p), 0))
         return U
+
+    def init_state(self, case, params: ICParams | None = None):
+        """The analytic initial state computed on the device (mhd_run_init_state): bitwise
+        ``set_state(initial_state(case, cfg, params))`` for the cases that have one
+        (Brio-Wu, shock-bubble, shock-cloud, the batch Orszag-Tang family)."""
+        check(lib().mhd_run_init_state(self._h, int(parse_case_id(case)),
+                                       C.byref(params) if params is not None else None))
 
     def set_state_device(self, ptr: int):
         """Copy from a dense device SoA buffer (9 x cells) at address ``ptr``."""

@@ -72,6 +72,8 @@ def orc():
         lib.orc_splitmix64.restype = C.c_uint64
         lib.orc_limiter.argtypes = [C.c_double, C.c_double, C.c_int]
         lib.orc_cell_uniform.argtypes = [C.c_uint64, C.c_int, C.c_int]
+        lib.orc_ic_sin.restype = C.c_double
+        lib.orc_ic_sin.argtypes = [C.c_double]
         lib.orc_fast_speed.argtypes = [_dp, C.c_int, C.c_double]
         lib.orc_max_signal_speed.argtypes = [_dp, C.c_int, C.c_double, C.c_double]
         for name in ("orc_cons_to_prim", "orc_prim_to_cons"):

@@ -9,6 +9,8 @@
 // tangent-linear model with branches frozen at the primal values.
 #include "mhd_oracle.hpp"
 
+#include "../mhd_gpu_acc_amd/csrc/ic.hpp"  // the batch family's initial state (ic_sin)
+
 #include <chrono>
 #include <cmath>
 #include <cstddef>
@@ -791,29 +793,20 @@ V9<double> ic_orszag_tang(double x, double y) {  // test_cases.cpp:139-148
                     gamma);
 }
 
-// Orszag-Tang on [0,1]^2 (rho = 25/(36 pi), p = 5/(12 pi), B scaled by
-// b_scale/sqrt(4 pi)), with an optional multi-mode density perturbation and
-// an optional Gaussian density bump (p held, E recomputed).
-V9<double> ic_ot_unit(double x, double y, double gamma, const orc_ic_params* p, bool modes, bool bump) {
+// Orszag-Tang on [0,1]^2 (rho = 25/(36 pi), p = 5/(12 pi), B = (-sin 2 pi y,
+// sin 4 pi x) / sqrt(4 pi)), with an optional Gaussian density bump (p held, E
+// recomputed).
+V9<double> ic_ot_unit(double x, double y, const orc_ic_params* p, bool bump) {
     double rho = 25.0 / (36.0 * kPi);
     const double pr = 5.0 / (12.0 * kPi);
-    // b_scale belongs to the batch family; the bump case keeps the standard field
-    const double b0 = (modes && p ? p->b_scale : 1.0) / std::sqrt(4.0 * kPi);
-    if (modes && p->n_modes > 0) {
-        double s = 0.0, norm = 0.0;
-        for (int m = 0; m < p->n_modes; ++m) {
-            s += p->mode_amp[m] * std::sin(2.0 * kPi * (p->mode_kx[m] * x + p->mode_ky[m] * y) + p->mode_phase[m]);
-            norm += std::fabs(p->mode_amp[m]);
-        }
-        rho = rho * (1.0 + p->pert_amp * (s / norm));
-    }
+    const double b0 = 1.0 / std::sqrt(4.0 * kPi);
     if (bump) {
         const double r2 = (x - p->bump_cx) * (x - p->bump_cx) + (y - p->bump_cy) * (y - p->bump_cy);
         rho = rho + p->bump_amp * std::exp(-r2 / (2.0 * p->bump_sigma * p->bump_sigma));
     }
     const double sy = std::sin(2.0 * kPi * y);
     const double sx = std::sin(2.0 * kPi * x);
-    return make_mhd(rho, -sy, sx, 0.0, -sy * b0, std::sin(4.0 * kPi * x) * b0, 0.0, pr, gamma);
+    return make_mhd(rho, -sy, sx, 0.0, -sy * b0, std::sin(4.0 * kPi * x) * b0, 0.0, pr, 5.0 / 3.0);
 }
 
 // MHD shock-cloud on [0,2]x[0,1]: post-shock slab x < 0.6, dense bubble r < 0.15
@@ -881,6 +874,7 @@ void orc_reconstruct(const double* Wm, const double* Wc, const double* Wp, doubl
 }
 uint64_t orc_splitmix64(uint64_t* state) { return splitmix(*state); }
 double orc_cell_uniform(uint64_t seed, int i, int j) { return cell_uniform(seed, i, j); }
+double orc_ic_sin(double a) { return mhd_ic::ic_sin(a); }
 
 int orc_compute_dt(const double* U, const orc_geom* gm, double cfl, double gamma, double* ch_inout, double* dt) {
     const Geo g(*gm);
@@ -1006,10 +1000,14 @@ int orc_init_state(int id, const orc_geom* gm, const orc_ic_params* p, double* U
                 case 1: s = ic_shock_bubble(x, y); break;
                 case 2: case 6: s = ic_brio_wu(x); break;
                 case 3: s = ic_orszag_tang(x, y); break;
-                case 4: s = ic_ot_unit(x, y, 5.0 / 3.0, nullptr, false, false); break;
+                case 4: s = ic_ot_unit(x, y, nullptr, false); break;
                 case 5: s = ic_shock_cloud(x, y, i, j, p ? p->seed : 0); break;
-                case 7: s = ic_ot_unit(x, y, p->gamma, p, true, false); break;
-                case 8: s = ic_ot_unit(x, y, 5.0 / 3.0, p, false, true); break;
+                case 7: {  // the batch family is this build's own case: one shared definition
+                    const mhd_ic::State f = mhd_ic::ot_family(x, y, *p);
+                    for (int k = 0; k < 9; ++k) s[k] = f.q[k];
+                    break;
+                }
+                case 8: s = ic_ot_unit(x, y, p, true); break;
                 default: return -1;
             }
             store9(s, U + 9 * g.at(i, j));

@@ -75,6 +75,8 @@ void   orc_reconstruct(const double* Wm, const double* Wc, const double* Wp, dou
                        double ch, double gamma, int limiter, double* UL_star, double* UR_star);
 uint64_t orc_splitmix64(uint64_t* state);
 double   orc_cell_uniform(uint64_t seed, int i, int j);
+// The portable sine of the batch family's initial state (ic.hpp, shared with the library).
+double   orc_ic_sin(double a);
 
 // ---- grids -----------------------------------------------------------------------------
 // Returns 0, or 1 when no interior cell has a finite positive signal speed.

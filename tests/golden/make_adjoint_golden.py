@@ -19,12 +19,13 @@ Two initial states:
   so no finite difference is well posed there.
 * "generic": the same state with seeded noise of relative size 1e-3
   (mhd_gpu_acc_amd.adjoint.generic_state(U0, 7)).  References: the tangent model at
-  the probes, and frozen-branch central differences at eps = 1e-6 and 1e-7 (the
-  oracle's forward code on a scalar that carries the perturbed copies through
-  every branch of the unperturbed run: finite differences with dt / c_h and every
-  branch code frozen).
+  the probes, and frozen-branch central differences D(eps) at eps = 1e-6, 1e-7 and
+  2e-6 (the oracle's forward code on a scalar that carries the perturbed copies
+  through every branch of the unperturbed run: finite differences with dt / c_h and
+  every branch code frozen).  D(1e-6) and D(2e-6) make the fourth-order central
+  difference at eps = 1e-6, (4 D(eps) - D(2 eps)) / 3.
 
-About 50 min on 8 cores.  Parts already present in an existing adjoint_ot512.json
+About 75 min on 8 cores.  Parts already present in an existing adjoint_ot512.json
 are kept unless --force is given.
 
     python tests/golden/make_adjoint_golden.py
@@ -116,7 +117,7 @@ def part(prev, U0, case, rho_t, pairs, cpu_adj, fd):
         res["cpu_adjoint"] = prev.get("cpu_adjoint") or cpu_adjoint(U0, case, run, resid, pairs, v)
     if fd:
         res["frozen_fd"] = dict(prev.get("frozen_fd", {}))
-        for eps in (1e-6, 1e-7):
+        for eps in (1e-6, 1e-7, 2e-6):
             key = f"{eps:.0e}"
             if key in res["frozen_fd"]:
                 continue

@@ -199,7 +199,9 @@ def test_adjoint_workload_ot512():
     on the Orszag-Tang state the gradient equals the oracle's tangent model at the 32
     seeded probes and on a whole-grid projection, and the CPU adjoint bit for bit; on
     the generic (noise-perturbed) state it also matches frozen-branch central
-    differences at eps = 1e-6 and 1e-7."""
+    differences: the fourth-order stencil at eps = 1e-6, (4 D(eps) - D(2 eps)) / 3, to
+    1e-6, and the two-point D(eps) at eps = 1e-6 and 1e-7 to the bound their
+    truncation error O(eps^2) allows."""
     import hashlib
     from mhd_gpu_acc_amd.adjoint import adjoint_workload, generic_state, to_planes
     if not os.path.exists(GOLDEN):
@@ -229,9 +231,15 @@ def test_adjoint_workload_ot512():
             assert float(np.sum(grad * v)) == pytest.approx(r["tangent"]["direction_dot"], rel=1e-9)
             if "cpu_adjoint" in r:
                 assert hashlib.sha256(np.ascontiguousarray(grad).tobytes()).hexdigest() == r["cpu_adjoint"]["sha256"]
+            ffd = r.get("frozen_fd", {})
             for key, bound in (("1e-06", 3e-6), ("1e-07", 3e-7)):
-                if key in r.get("frozen_fd", {}):
-                    fd = np.array(r["frozen_fd"][key])
+                if key in ffd:
+                    fd = np.array(ffd[key])
                     nrel = np.linalg.norm(adj - fd) / np.linalg.norm(fd)
                     say(f"  {name}: vs frozen-branch central differences eps {key}: norm rel {nrel:.3e}")
                     assert nrel <= bound
+            if "1e-06" in ffd and "2e-06" in ffd:
+                fd4 = (4.0 * np.array(ffd["1e-06"]) - np.array(ffd["2e-06"])) / 3.0
+                nrel = np.linalg.norm(adj - fd4) / np.linalg.norm(fd4)
+                say(f"  {name}: vs fourth-order frozen-branch central differences eps 1e-6: norm rel {nrel:.3e}")
+                assert nrel <= 1e-6

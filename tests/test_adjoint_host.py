@@ -337,3 +337,11 @@ def test_adjoint_workload_fixture_consistent():
     e6 = np.linalg.norm(np.array(ge["frozen_fd"]["1e-06"]) - gt) / np.linalg.norm(gt)
     e7 = np.linalg.norm(np.array(ge["frozen_fd"]["1e-07"]) - gt) / np.linalg.norm(gt)
     assert e6 <= 3e-6 and e7 <= 3e-7 and e7 < e6 / 5, (e6, e7)
+    if "2e-06" in ge["frozen_fd"]:
+        # D(2 eps) - D(eps) ~ 3 c eps^2: the two-point error is the truncation term,
+        # which the fourth-order combination removes
+        d1, d2 = np.array(ge["frozen_fd"]["1e-06"]), np.array(ge["frozen_fd"]["2e-06"])
+        e2 = np.linalg.norm(d2 - gt) / np.linalg.norm(gt)
+        e4 = np.linalg.norm((4.0 * d1 - d2) / 3.0 - gt) / np.linalg.norm(gt)
+        assert 3.0 < e2 / e6 < 5.0, (e2, e6)
+        assert e4 <= 1e-7, e4
