@@ -163,7 +163,7 @@ int mhd_run_create(const mhd_run_desc* desc, mhd_run** out) {
     const char* xgrid = std::getenv("MHD_XGRID");
     r->xgrid = xgrid ? std::atoi(xgrid) : 8;  // persistent x sweep, 8 blocks per CU (measured best)
     int dev = 0;
-    hipGetDevice(&dev);
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
     if (hipDeviceGetAttribute(&r->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || r->cus < 1)
         r->cus = 1;
     auto cleanup = [&](hipError_t e, const char* what) {
@@ -188,13 +188,13 @@ int mhd_run_create(const mhd_run_desc* desc, mhd_run** out) {
 
 int mhd_run_destroy(mhd_run* r) {
     if (!r) return MHD_OK;
-    hipStreamSynchronize(r->stream());
-    for (hipEvent_t ev : r->events) hipEventDestroy(ev);
+    (void)hipStreamSynchronize(r->stream());
+    for (hipEvent_t ev : r->events) (void)hipEventDestroy(ev);
     for (double* p : r->U)
-        if (p) hipFree(p);
-    if (r->frame) hipFree(r->frame);
-    if (r->acc) hipFree(r->acc);
-    if (r->host_acc) hipHostFree(r->host_acc);
+        if (p) (void)hipFree(p);
+    if (r->frame) (void)hipFree(r->frame);
+    if (r->acc) (void)hipFree(r->acc);
+    if (r->host_acc) (void)hipHostFree(r->host_acc);
     delete r;
     return MHD_OK;
 }
@@ -364,7 +364,7 @@ int mhd_advance_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew
 int mhd_profile(mhd_run* r, int enable) {
     if (!r) return fail(MHD_ERR_INVALID, "null run");
     MHD_HIP(hipStreamSynchronize(r->stream()));
-    for (hipEvent_t ev : r->events) hipEventDestroy(ev);
+    for (hipEvent_t ev : r->events) (void)hipEventDestroy(ev);
     r->events.clear();
     r->prof_steps = 0;
     r->profiling = enable != 0;

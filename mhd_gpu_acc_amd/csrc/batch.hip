@@ -192,7 +192,7 @@ namespace {
     } while (0)
 
 template <class T> int upload(T** dptr, const std::vector<T>& v) {
-    if (*dptr) hipFree(*dptr);
+    if (*dptr) (void)hipFree(*dptr);
     *dptr = nullptr;
     if (v.empty()) return MHD_OK;
     B_HIP(hipMalloc(dptr, v.size() * sizeof(T)));
@@ -266,8 +266,8 @@ hipError_t launch_batch_step(mhd_batch* b) {
 }
 
 void drop_graph(mhd_batch* b) {
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
 }
@@ -283,7 +283,7 @@ int capture_graph(mhd_batch* b) {
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(b->stream, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return set_error(MHD_ERR_HIP, (std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2)).c_str());
     }
     b->graph = g;
@@ -363,7 +363,7 @@ int mhd_batch_create(const mhd_run_desc* descs, int n, mhd_batch** out) {
     b->d.assign(descs, descs + n);
     b->ch.assign(n, 0.0);
     int dev = 0;
-    hipGetDevice(&dev);
+    if (hipGetDevice(&dev) != hipSuccess) return set_error(MHD_ERR_HIP, "hipGetDevice");
     if (hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || b->cus < 1)
         b->cus = 1;
     auto bail = [&](int rc) {
@@ -390,7 +390,8 @@ int mhd_batch_create(const mhd_run_desc* descs, int n, mhd_batch** out) {
         for (int k = 0; k < 3; ++k) {
             if (hipMalloc(&R.U[k], 9 * R.ps * sizeof(double)) != hipSuccess)
                 return bail(set_error(MHD_ERR_HIP, "hipMalloc(batch grid)"));
-            hipMemsetAsync(R.U[k], 0, 9 * R.ps * sizeof(double), b->stream);
+            if (hipMemsetAsync(R.U[k], 0, 9 * R.ps * sizeof(double), b->stream) != hipSuccess)
+                return bail(set_error(MHD_ERR_HIP, "hipMemsetAsync(batch grid)"));
         }
         if (hipMalloc(&R.frame, 4 * (size_t)(d.nx + d.ny) * sizeof(double)) != hipSuccess)
             return bail(set_error(MHD_ERR_HIP, "hipMalloc(frame)"));
@@ -411,19 +412,19 @@ int mhd_batch_create(const mhd_run_desc* descs, int n, mhd_batch** out) {
 
 int mhd_batch_destroy(mhd_batch* b) {
     if (!b) return MHD_OK;
-    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graph(b);
     for (BatchRun& R : b->runs) {
         for (double* p : R.U)
-            if (p) hipFree(p);
-        if (R.frame) hipFree(R.frame);
+            if (p) (void)hipFree(p);
+        if (R.frame) (void)hipFree(R.frame);
     }
     for (void* p : {(void*)b->d_runs, (void*)b->d_steps, (void*)b->d_acc, (void*)b->d_queue, (void*)b->d_xt, (void*)b->d_yt,
                     (void*)b->d_bt, (void*)b->d_ct})
-        if (p) hipFree(p);
-    if (b->h_steps) hipHostFree(b->h_steps);
-    if (b->h_acc) hipHostFree(b->h_acc);
-    if (b->own_stream) hipStreamDestroy(b->stream);
+        if (p) (void)hipFree(p);
+    if (b->h_steps) (void)hipHostFree(b->h_steps);
+    if (b->h_acc) (void)hipHostFree(b->h_acc);
+    if (b->own_stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return MHD_OK;
 }
@@ -593,15 +594,16 @@ int mhd_batch_tune(mhd_batch* b, int reps, char* report, size_t report_len) {
                     float total = 0.0f;
                     for (int k = 0; k <= reps; ++k) {
                         for (int r = 0; r < b->n; ++r) b->h_steps[r].cur = b->cur;
-                        hipEventRecord(e0, b->stream);
-                        hipGraphLaunch(b->exec, b->stream);
-                        hipEventRecord(e1, b->stream);
-                        if (hipStreamSynchronize(b->stream) != hipSuccess) {
-                            rc = set_error(MHD_ERR_HIP, "tune replay");
+                        float ms = 0.0f;
+                        hipError_t e = hipEventRecord(e0, b->stream);
+                        if (e == hipSuccess) e = hipGraphLaunch(b->exec, b->stream);
+                        if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+                        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+                        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                        if (e != hipSuccess) {
+                            rc = set_error(MHD_ERR_HIP, (std::string("tune replay: ") + hipGetErrorString(e)).c_str());
                             goto done;
                         }
-                        float ms = 0.0f;
-                        hipEventElapsedTime(&ms, e0, e1);
                         if (k > 0) total += ms;  // first replay warms up
                         b->cur = 2 - b->cur;
                     }
@@ -615,17 +617,20 @@ int mhd_batch_tune(mhd_batch* b, int reps, char* report, size_t report_len) {
                     }
                 }
 done:
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     // Restore the states and the step bookkeeping, keep the fastest shape.
     b->cur = cur0;
-    for (int r = 0; r < b->n; ++r) {
+    hipError_t er = hipSuccess;
+    for (int r = 0; r < b->n && er == hipSuccess; ++r) {
         const size_t bytes = 9 * (size_t)b->runs[r].ps * sizeof(double);
-        hipMemcpyAsync(b->runs[r].U[b->cur], save[r], bytes, hipMemcpyDeviceToDevice, b->stream);
+        er = hipMemcpyAsync(b->runs[r].U[b->cur], save[r], bytes, hipMemcpyDeviceToDevice, b->stream);
     }
-    hipStreamSynchronize(b->stream);
+    if (er == hipSuccess) er = hipStreamSynchronize(b->stream);
+    if (er != hipSuccess && rc == MHD_OK)
+        rc = set_error(MHD_ERR_HIP, (std::string("tune restore: ") + hipGetErrorString(er)).c_str());
     for (double* p : save)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     b->ch = ch0;
     std::memcpy(b->h_acc, acc0.data(), b->n * sizeof(unsigned long long));
     b->acc_valid = true;

@@ -139,13 +139,13 @@ extern "C" {
 
 int mhd_tape_destroy(mhd_tape* t) {
     if (!t) return MHD_OK;
-    if (t->run) hipStreamSynchronize(t->run->stream());
-    for (double* p : t->ck) if (p) hipFree(p);
-    for (double* p : t->S) if (p) hipFree(p);
-    for (double* p : t->T) if (p) hipFree(p);
-    for (double* p : t->lam) if (p) hipFree(p);
-    if (t->count) hipFree(t->count);
-    if (t->host_count) hipHostFree(t->host_count);
+    if (t->run) (void)hipStreamSynchronize(t->run->stream());
+    for (double* p : t->ck) if (p) (void)hipFree(p);
+    for (double* p : t->S) if (p) (void)hipFree(p);
+    for (double* p : t->T) if (p) (void)hipFree(p);
+    for (double* p : t->lam) if (p) (void)hipFree(p);
+    if (t->count) (void)hipFree(t->count);
+    if (t->host_count) (void)hipHostFree(t->host_count);
     delete t;
     return MHD_OK;
 }
