@@ -207,8 +207,8 @@ int mhd_batch_plan_ot_family(uint64_t seed, int n, int* sizes, mhd_ic_params* pa
  * compute_dt + advance_second_order.  dt and c_h are frozen at the recorded
  * values and every branch (Riemann-solver region and fallbacks, limiter
  * selections, physical-state guards) at the one the recorded trajectory takes.
- * Supported: HLL and HLLD, minmod and van Leer, periodic / transmissive /
- * reflecting sides (MHD_ERR_INVALID otherwise).  The gradient is with respect to
+ * Supported: every Riemann solver (HLL, HLLC, HLLD, FORCE) and limiter, periodic /
+ * transmissive / reflecting sides (MHD_ERR_INVALID for Dirichlet).  The gradient is with respect to
  * every value of U_0, ghost cells included. */
 typedef struct mhd_tape mhd_tape;
 /* Device memory: ceil(nsteps/K) checkpoints, 2K+1 segment grids, 2 adjoint grids. */

@@ -9,8 +9,8 @@ the hand-written adjoint kernels: each K-step segment is recomputed from its
 checkpoint, then reversed.  dt and c_h are frozen at the values the forward
 recorded, and every branch (Riemann regions and fallbacks, limiter selections,
 physical-state guards) at the one the forward took; the gradient is with respect
-to every value of U0, ghost cells included.  HLL or HLLD, minmod or van Leer,
-periodic / transmissive / reflecting sides.
+to every value of U0, ghost cells included.  Every Riemann solver (HLL, HLLC,
+HLLD, FORCE) and limiter; periodic / transmissive / reflecting sides.
 
 Nothing here computes on the host: both directions are kernels of libmhd_hip.so.
 """

@@ -181,8 +181,12 @@ hipError_t launch_adjoint(const AdjArgs& a) {
     switch (a.solver * 2 + a.limiter) {
         case 0: return launch_adjoint_sl<0, 0>(a);
         case 1: return launch_adjoint_sl<0, 1>(a);
+        case 2: return launch_adjoint_sl<1, 0>(a);
+        case 3: return launch_adjoint_sl<1, 1>(a);
         case 4: return launch_adjoint_sl<2, 0>(a);
         case 5: return launch_adjoint_sl<2, 1>(a);
+        case 6: return launch_adjoint_sl<3, 0>(a);
+        case 7: return launch_adjoint_sl<3, 1>(a);
     }
     return hipErrorInvalidValue;
 }

@@ -612,21 +612,264 @@ MHD_DI int hlld_adj(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& 
     return 1;
 }
 
-// Adjoint of riemann<SOLVER, XDIR>() for SOLVER 0 (HLL) and 2 (HLLD).  Bars of the
-// two interface states are accumulated as (cons, prim) pairs.
+// f = force(ul, vl, fl, ur, vr, fr): half the Lax-Friedrichs flux with speed a plus
+// half the flux of the Richtmyer state uri (physics.hpp force).
+MHD_DI void force_adj(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur, const Prim& vr, const Cons& fr,
+                      const Phys& ph, const Cons& fb, Cons& ulb, Prim& vlb, Cons& flb, Cons& urb, Prim& vrb,
+                      Cons& frb) {
+    if (!prim_physical(vl) || !prim_physical(vr)) {
+        acc_scaled(flb, 0.5, fb);
+        acc_scaled(frb, 0.5, fb);
+        return;
+    }
+    const FastSpeed tl = fast_speed_fwd(vl, ph);
+    const FastSpeed tr = fast_speed_fwd(vr, ph);
+    const double A = fabs(vl.un) + tl.c, B = fabs(vr.un) + tr.c;
+    const double m = fmax(A, B);
+    const double a = fmax(m, ph.ch);
+    if (a < 1.0e-14) {
+        acc_scaled(flb, 0.5, fb);
+        acc_scaled(frb, 0.5, fb);
+        return;
+    }
+    const double ha = 0.5 * a;
+    const double hia = 0.5 * (1.0 / a);
+    const Cons uri{0.5 * (ul.r + ur.r) - hia * (fr.r - fl.r),       0.5 * (ul.mn + ur.mn) - hia * (fr.mn - fl.mn),
+                   0.5 * (ul.mt + ur.mt) - hia * (fr.mt - fl.mt),   0.5 * (ul.mz + ur.mz) - hia * (fr.mz - fl.mz),
+                   0.5 * (ul.bn + ur.bn) - hia * (fr.bn - fl.bn),   0.5 * (ul.bt + ur.bt) - hia * (fr.bt - fl.bt),
+                   0.5 * (ul.bz + ur.bz) - hia * (fr.bz - fl.bz),   0.5 * (ul.e + ur.e) - hia * (fr.e - fl.e),
+                   0.5 * (ul.psi + ur.psi) - hia * (fr.psi - fl.psi)};
+    const Prim vri = to_prim(uri, ph);
+    Cons flfb = fb, urib = cons_zero();
+    if (prim_physical(vri)) {
+        // f = 0.5 (flf + flux(uri, vri))
+        flfb = cons_zero();
+        acc_scaled(flfb, 0.5, fb);
+        Cons fxb = cons_zero();
+        acc_scaled(fxb, 0.5, fb);
+        Prim vrib = prim_zero();
+        flux_adj(uri, vri, fxb, ph, urib, vrib);
+        to_prim_adj(uri, vri, vrib, ph, urib);
+    }
+    double hab = 0.0, hiab = 0.0;
+#define MHD_FORCE_ADJ(f)                               \
+    {                                                  \
+        flb.f += 0.5 * flfb.f;                         \
+        frb.f += 0.5 * flfb.f;                         \
+        urb.f -= ha * flfb.f;                          \
+        ulb.f += ha * flfb.f;                          \
+        hab -= (ur.f - ul.f) * flfb.f;                 \
+        ulb.f += 0.5 * urib.f;                         \
+        urb.f += 0.5 * urib.f;                         \
+        frb.f -= hia * urib.f;                         \
+        flb.f += hia * urib.f;                         \
+        hiab -= (fr.f - fl.f) * urib.f;                \
+    }
+    MHD_FORCE_ADJ(r) MHD_FORCE_ADJ(mn) MHD_FORCE_ADJ(mt) MHD_FORCE_ADJ(mz) MHD_FORCE_ADJ(bn) MHD_FORCE_ADJ(bt)
+    MHD_FORCE_ADJ(bz) MHD_FORCE_ADJ(e) MHD_FORCE_ADJ(psi)
+#undef MHD_FORCE_ADJ
+    // ha = 0.5 a, hia = 0.5 (1/a); a = fmax(fmax(A, B), c_h)
+    const double ab = 0.5 * hab - 0.5 * hiab / (a * a);
+    if (!picks_a(a, m)) return;  // a = c_h, a constant
+    double cflb = 0.0, cfrb = 0.0;
+    if (picks_a(m, A)) {
+        vlb.un += sign0(vl.un) * ab;
+        cflb = ab;
+    } else {
+        vrb.un += sign0(vr.un) * ab;
+        cfrb = ab;
+    }
+    fast_speed_adj(vl, tl, cflb, ph, vlb);
+    fast_speed_adj(vr, tr, cfrb, ph, vrb);
+}
+
+// Adjoint of the HLLC flux as riemann() uses it (physics.hpp hllc), given the bar of
+// riemann()'s result.  Return codes as hlld_adj: 0 when hllc() falls back to HLL, 2
+// when its flux fails riemann()'s finiteness test (nothing accumulated in both), 1
+// otherwise (the GLM overwrite and the HLLC flux are reversed).
+template <bool XDIR>
+MHD_DI int hllc_adj(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr, const Phys& ph, Cons fb,
+                    Cons& ulb, Prim& vlb, Cons& urb, Prim& vrb) {
+    if (!prim_physical(vl) || !prim_physical(vr)) return 0;
+    const FastSpeed tl = fast_speed_fwd(vl, ph);
+    const FastSpeed tr = fast_speed_fwd(vr, ph);
+    const double sl = fmin(vl.un - tl.c, vr.un - tr.c);
+    const double sr = fmax(vl.un + tl.c, vr.un + tr.c);
+    if (sl >= 0.0 || sr <= 0.0) {
+        const bool l = sl >= 0.0;
+        const Cons f = l ? flux(ul, vl, ph) : flux(ur, vr, ph);
+        const double fbx = XDIR ? f.bn : f.bt;
+        if (!finite(f.r) || !finite(f.e) || !finite(fbx) || !finite(f.psi)) return 2;
+        glm_overwrite_adj(fb, ph, ulb, urb);
+        if (l) flux_adj(ul, vl, fb, ph, ulb, vlb);
+        else flux_adj(ur, vr, fb, ph, urb, vrb);
+        return 1;
+    }
+    const double ptl = vl.p + 0.5 * vl.b2;
+    const double ptr = vr.p + 0.5 * vr.b2;
+    const double a1 = sl - vl.un;
+    const double b1 = sr - vr.un;
+    const double num = b1 * vr.r * vr.un - a1 * vl.r * vl.un + ptl - ptr;
+    const double den = b1 * vr.r - a1 * vl.r;
+    if (fabs(den) < 1.0e-14) return 0;
+    const double sm = num / den;
+    const double yl = sl - sm, yr = sr - sm;
+    const double rls = vl.r * a1 / yl;
+    const double rrs = vr.r * b1 / yr;
+    if (rls <= 0.0 || rrs <= 0.0 || !finite(rls) || !finite(rrs)) return 0;
+    const double psim = 0.5 * (vl.psi + vr.psi) - ph.half_ch * (vr.bn - vl.bn);
+    const bool left = sm >= 0.0;
+    const Cons& u = left ? ul : ur;
+    const Prim& v = left ? vl : vr;
+    const double s = left ? sl : sr;
+    const double pt = left ? ptl : ptr;
+    const double rs = left ? rls : rrs;
+    const double c1 = s - v.un, c2 = sm - v.un;
+    const double pts = pt + v.r * c1 * c2;
+    const double bu = v.bn * v.un + v.bt * v.ut + v.bz * v.uz;
+    const double bus = v.bn * sm + v.bt * v.ut + v.bz * v.uz;
+    const double ys = s - sm;
+    const double Ne = c1 * u.e - pt * v.un + pts * sm + v.bn * (bus - bu);
+    const double es = Ne / ys;
+    const Cons us = star(rs, sm, v.ut, v.uz, v.bn, v.bt, v.bz, es, psim);
+    const Cons f = fan(flux(u, v, ph), s, us, u);
+    {
+        const double fbx = XDIR ? f.bn : f.bt;
+        if (!finite(f.r) || !finite(f.e) || !finite(fbx) || !finite(f.psi)) return 2;
+    }
+    glm_overwrite_adj(fb, ph, ulb, urb);
+
+    // ---- reverse ----  f = flux(u, v) + s (us - u)
+    Cons& ub = left ? ulb : urb;
+    Prim& vb = left ? vlb : vrb;
+    double sb = dot(fb, diff(us, u));
+    Cons usb = cons_zero();
+    acc_scaled(usb, s, fb);
+    acc_scaled(ub, -s, fb);
+    flux_adj(u, v, fb, ph, ub, vb);
+    StarBar st{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    star_adj(rs, sm, v.ut, v.uz, usb, st);
+    vb.ut += st.ut;
+    vb.uz += st.uw;
+    vb.bn += st.bn;
+    vb.bt += st.bt;
+    vb.bz += st.bw;
+    double smb = st.sm, ptb = 0.0;
+    // es = Ne / ys, ys = s - sm
+    const double Neb = st.e / ys;
+    const double ysb = -es * Neb;
+    sb += ysb;
+    smb -= ysb;
+    // Ne = c1 u.e - pt v.un + pts sm + v.bn (bus - bu)
+    double c1b = u.e * Neb, c2b = 0.0;
+    ub.e += c1 * Neb;
+    ptb -= v.un * Neb;
+    vb.un -= pt * Neb;
+    const double ptsb = sm * Neb;
+    smb += pts * Neb;
+    vb.bn += (bus - bu) * Neb;
+    const double busb = v.bn * Neb, bub = -v.bn * Neb;
+    // bus = v.bn sm + v.bt v.ut + v.bz v.uz;  bu = v.bn v.un + v.bt v.ut + v.bz v.uz
+    vb.bn += sm * busb + v.un * bub;
+    smb += v.bn * busb;
+    vb.un += v.bn * bub;
+    vb.bt += v.ut * (busb + bub);
+    vb.ut += v.bt * (busb + bub);
+    vb.bz += v.uz * (busb + bub);
+    vb.uz += v.bz * (busb + bub);
+    // pts = pt + v.r c1 c2
+    ptb += ptsb;
+    vb.r += c1 * c2 * ptsb;
+    c1b += v.r * c2 * ptsb;
+    c2b += v.r * c1 * ptsb;
+    // c1 = s - v.un, c2 = sm - v.un
+    sb += c1b;
+    vb.un -= c1b + c2b;
+    smb += c2b;
+    // psim = 0.5 (vl.psi + vr.psi) - half_ch (vr.bn - vl.bn)
+    vlb.psi += 0.5 * st.psi;
+    vrb.psi += 0.5 * st.psi;
+    vrb.bn -= ph.half_ch * st.psi;
+    vlb.bn += ph.half_ch * st.psi;
+    double slb = left ? sb : 0.0, srb = left ? 0.0 : sb;
+    double ptlb = left ? ptb : 0.0, ptrb = left ? 0.0 : ptb;
+    const double rlsb = left ? st.rs : 0.0, rrsb = left ? 0.0 : st.rs;
+    // rls = vl.r a1 / yl, rrs = vr.r b1 / yr
+    double a1b = 0.0, b1b = 0.0;
+    {
+        const double xlb = rlsb / yl;
+        const double ylb = -rls * xlb;
+        vlb.r += a1 * xlb;
+        a1b += vl.r * xlb;
+        slb += ylb;
+        smb -= ylb;
+        const double xrb = rrsb / yr;
+        const double yrb = -rrs * xrb;
+        vrb.r += b1 * xrb;
+        b1b += vr.r * xrb;
+        srb += yrb;
+        smb -= yrb;
+    }
+    // sm = num / den
+    const double numb = smb / den;
+    const double denb = -sm * numb;
+    b1b += vr.r * vr.un * numb;
+    vrb.r += b1 * vr.un * numb;
+    vrb.un += b1 * vr.r * numb;
+    a1b -= vl.r * vl.un * numb;
+    vlb.r -= a1 * vl.un * numb;
+    vlb.un -= a1 * vl.r * numb;
+    ptlb += numb;
+    ptrb -= numb;
+    b1b += vr.r * denb;
+    vrb.r += b1 * denb;
+    a1b -= vl.r * denb;
+    vlb.r -= a1 * denb;
+    slb += a1b;
+    vlb.un -= a1b;
+    srb += b1b;
+    vrb.un -= b1b;
+    vlb.p += ptlb;
+    vlb.b2 += 0.5 * ptlb;
+    vrb.p += ptrb;
+    vrb.b2 += 0.5 * ptrb;
+    wave_speeds_adj(vl, vr, tl, tr, ph, false, slb, srb, vlb, vrb);
+    return 1;
+}
+
+// Adjoint of riemann<SOLVER, XDIR>() for every solver: 0 HLL, 1 HLLC, 2 HLLD, 3 FORCE.
+// Bars of the two interface states are accumulated as (cons, prim) pairs.
 template <int SOLVER, bool XDIR>
 MHD_DI void riemann_adj(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr, const Phys& ph,
                         const Cons& fb, Cons& ulb, Prim& vlb, Cons& urb, Prim& vrb) {
-    static_assert(SOLVER == 0 || SOLVER == 2, "adjoint: HLL and HLLD only");
-    if (SOLVER == 2) {
-        const int code = hlld_adj<XDIR>(ul, vl, ur, vr, ph, fb, ulb, vlb, urb, vrb);
+    static_assert(SOLVER >= 0 && SOLVER <= 3, "adjoint: unknown solver");
+    if (SOLVER == 3) {
+        const Cons fl = flux(ul, vl, ph);
+        const Cons fr = flux(ur, vr, ph);
+        Cons flb = cons_zero(), frb = cons_zero();
+        const Cons f = force(ul, vl, fl, ur, vr, fr, ph);
+        const double fbx = XDIR ? f.bn : f.bt;
+        if (!finite(f.r) || !finite(f.e) || !finite(fbx) || !finite(f.psi)) {
+            hll_adj(ul, vl, fl, ur, vr, fr, ph, fb, ulb, vlb, flb, urb, vrb, frb);  // riemann()'s HLL fallback
+        } else {
+            Cons g = fb;
+            glm_overwrite_adj(g, ph, ulb, urb);
+            force_adj(ul, vl, fl, ur, vr, fr, ph, g, ulb, vlb, flb, urb, vrb, frb);
+        }
+        flux_adj(ul, vl, flb, ph, ulb, vlb);
+        flux_adj(ur, vr, frb, ph, urb, vrb);
+        return;
+    }
+    if (SOLVER == 1 || SOLVER == 2) {
+        const int code = SOLVER == 1 ? hllc_adj<XDIR>(ul, vl, ur, vr, ph, fb, ulb, vlb, urb, vrb)
+                                     : hlld_adj<XDIR>(ul, vl, ur, vr, ph, fb, ulb, vlb, urb, vrb);
         if (code == 1) return;
         const Cons fl = flux(ul, vl, ph);
         const Cons fr = flux(ur, vr, ph);
         Cons flb = cons_zero(), frb = cons_zero();
         Cons g = fb;
         if (code == 0) {
-            // HLL fallback of hlld(): finiteness test, then the GLM overwrite
+            // HLL fallback of hllc()/hlld(): finiteness test, then the GLM overwrite
             const Cons f = hll(ul, vl, fl, ur, vr, fr, ph);
             const double fbx = XDIR ? f.bn : f.bt;
             if (finite(f.r) && finite(f.e) && finite(fbx) && finite(f.psi)) glm_overwrite_adj(g, ph, ulb, urb);

@@ -129,8 +129,15 @@ void hc_riemann_adj(const double* UL, const double* UR, int dir, int solver, dou
     // the face bar is Fb itself.
     const Cons fb = rot(Fb, x), z = cons_zero();
     Cons lb, rb;
-    if (solver == 0) { if (x) adj_face<0, true>(l, r, z, fb, 1.0, ph, lb, rb); else adj_face<0, false>(l, r, z, fb, 1.0, ph, lb, rb); }
-    else { if (x) adj_face<2, true>(l, r, z, fb, 1.0, ph, lb, rb); else adj_face<2, false>(l, r, z, fb, 1.0, ph, lb, rb); }
+    switch (solver) {
+#define MHD_HC_FACE(S)                                        \
+    case S:                                                   \
+        if (x) adj_face<S, true>(l, r, z, fb, 1.0, ph, lb, rb); \
+        else adj_face<S, false>(l, r, z, fb, 1.0, ph, lb, rb);  \
+        break;
+        MHD_HC_FACE(0) MHD_HC_FACE(1) MHD_HC_FACE(2) MHD_HC_FACE(3)
+#undef MHD_HC_FACE
+    }
     unrot(lb, x, ULb);
     unrot(rb, x, URb);
 }
@@ -214,8 +221,12 @@ int hc_adjoint_step(const double* Uold, const double* Utmp, int nx, int ny, doub
     switch (solver * 2 + limiter) {
         case 0: reverse_step<0, 0>(Uold, Utmp, lam, g, dx, dy, dt, ch, cr, gamma, bc); return 0;
         case 1: reverse_step<0, 1>(Uold, Utmp, lam, g, dx, dy, dt, ch, cr, gamma, bc); return 0;
+        case 2: reverse_step<1, 0>(Uold, Utmp, lam, g, dx, dy, dt, ch, cr, gamma, bc); return 0;
+        case 3: reverse_step<1, 1>(Uold, Utmp, lam, g, dx, dy, dt, ch, cr, gamma, bc); return 0;
         case 4: reverse_step<2, 0>(Uold, Utmp, lam, g, dx, dy, dt, ch, cr, gamma, bc); return 0;
         case 5: reverse_step<2, 1>(Uold, Utmp, lam, g, dx, dy, dt, ch, cr, gamma, bc); return 0;
+        case 6: reverse_step<3, 0>(Uold, Utmp, lam, g, dx, dy, dt, ch, cr, gamma, bc); return 0;
+        case 7: reverse_step<3, 1>(Uold, Utmp, lam, g, dx, dy, dt, ch, cr, gamma, bc); return 0;
     }
     return 1;
 }

@@ -92,8 +92,7 @@ int download(mhd_run* r, const double* src, int layout, double* dst) {
 }
 
 int check_adjoint_run(const mhd_run* r) {
-    if (r->d.solver != MHD_HLL && r->d.solver != MHD_HLLD)
-        return fail(MHD_ERR_INVALID, "adjoint: the Riemann solver must be HLL or HLLD");
+    if (r->d.solver < MHD_HLL || r->d.solver > MHD_FORCE) return fail(MHD_ERR_INVALID, "adjoint: unknown Riemann solver");
     for (int k = 0; k < 4; ++k)
         if (r->d.bc[k] == MHD_BC_DIRICHLET) return fail(MHD_ERR_INVALID, "adjoint: Dirichlet boundaries are not supported");
     return MHD_OK;

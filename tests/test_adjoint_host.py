@@ -81,7 +81,7 @@ def riemann_pairs(rng, direction):
 
 
 @pytest.mark.parametrize("direction", [0, 1])
-@pytest.mark.parametrize("solver", [0, 2])
+@pytest.mark.parametrize("solver", [0, 1, 2, 3])
 def test_riemann_adjoint_dot(direction, solver):
     rng = np.random.default_rng(11 + direction + 10 * solver)
     for ch in (0.0, 0.7, 3.0):
@@ -207,7 +207,7 @@ BCS = [(0, 0, 0, 0), (1, 1, 1, 1), (2, 2, 2, 2), (0, 0, 2, 1), (2, 1, 0, 0), (1,
 
 
 @pytest.mark.parametrize("bc", BCS)
-@pytest.mark.parametrize("solver,limiter", [(2, 1), (2, 0), (0, 1), (0, 0)])
+@pytest.mark.parametrize("solver,limiter", [(2, 1), (2, 0), (0, 1), (0, 0), (1, 1), (1, 0), (3, 1), (3, 0)])
 def test_step_adjoint_dot(bc, solver, limiter):
     case = small_case(13, 10, bc)
     U = perturbed_ot(case, 100 + bc[0] + 3 * bc[2])
@@ -237,7 +237,8 @@ def test_step_adjoint_ghost_rows_and_corners_zero():
     assert np.any(A[2:-2, :2] != 0)  # x ghosts are read by the x sweep
 
 
-@pytest.mark.parametrize("solver,limiter,bc", [(2, 1, (0, 0, 0, 0)), (0, 0, (1, 2, 0, 0))])
+@pytest.mark.parametrize("solver,limiter,bc", [(2, 1, (0, 0, 0, 0)), (0, 0, (1, 2, 0, 0)), (1, 1, (0, 0, 2, 2)),
+                                               (3, 0, (2, 1, 0, 0))])
 def test_multistep_adjoint_dot(solver, limiter, bc):
     """Five steps with the c_h ratchet and damping: reverse steps chained over the
     stored states against the tangent run with the recorded dt / c_h."""
@@ -269,8 +270,8 @@ def test_unsupported_configurations_rejected():
     U = perturbed_ot(small_case(6, 6, (0, 0, 0, 0)), 1)
     with pytest.raises(ValueError):
         hc.adjoint_step(U, U, case, 1e-3, 1.0, 2, 1, U)
-    with pytest.raises(ValueError):
-        hc.adjoint_step(U, U, small_case(6, 6, (0, 0, 0, 0)), 1e-3, 1.0, 1, 1, U)
+    with pytest.raises(ValueError):  # no solver 4
+        hc.adjoint_step(U, U, small_case(6, 6, (0, 0, 0, 0)), 1e-3, 1.0, 4, 1, U)
 
 
 def test_frozen_branch_differences_match_tangent():
@@ -301,7 +302,7 @@ def test_frozen_branch_differences_match_tangent():
     assert np.all(best <= 1e-6), best
 
 
-@pytest.mark.parametrize("solver,limiter", [(2, 1), (0, 0)])
+@pytest.mark.parametrize("solver,limiter", [(2, 1), (0, 0), (1, 1), (3, 0)])
 def test_step_adjoint_dot_uniform_pressure(solver, limiter):
     """The unperturbed Orszag-Tang state: uniform density and pressure, the latter with
     rounding noise from the conserved variables, so the rounding-level slope rule of
