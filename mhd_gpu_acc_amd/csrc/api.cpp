@@ -81,7 +81,7 @@ int dt_from_state(mhd_run* r, const double* U, bool reuse, double* dt) {
     return MHD_OK;
 }
 
-int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt) {
+int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt, bool want_cfl) {
     mhd::StepArgs a{};
     a.Uold = Uold;
     a.Utmp = Utmp;
@@ -104,7 +104,7 @@ int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, doubl
     a.xgrid = r->xgrid;
     a.cus = r->cus;
     a.frame = r->frame;
-    a.acc = r->acc;
+    a.acc = want_cfl ? r->acc : nullptr;
     a.stream = r->stream();
     a.ev = nullptr;
     if (r->profiling) {

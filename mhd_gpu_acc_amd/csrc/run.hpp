@@ -39,7 +39,9 @@ int hip_fail(hipError_t e, const char* what);
 // compute_dt on grid U (reuse: the fused CFL maximum of the last step is current).
 int dt_from_state(mhd_run* r, const double* U, bool reuse, double* dt);
 // One advance_second_order from Uold through Utmp into Unew with the run's c_h.
-int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt);
+// want_cfl: the y sweep also reduces Unew's CFL maximum into the run's accumulator
+// (the next compute_dt); a replay with recorded dt / c_h skips it.
+int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt, bool want_cfl = true);
 
 }  // namespace mhd
 

@@ -243,7 +243,7 @@ int mhd_tape_backward(mhd_tape* t, const double* lam_final, double* lam0, int la
             MHD_HIP(hipMemcpyAsync(t->S[0], t->ck[k], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
             for (int q = 0; q < m && !rc; ++q) {
                 r->ch = t->chs[s0 + q];
-                rc = mhd::step_grids(r, t->S[q], t->T[q], t->S[q + 1], t->dts[s0 + q]);
+                rc = mhd::step_grids(r, t->S[q], t->T[q], t->S[q + 1], t->dts[s0 + q], false);
             }
         }
         if (rc) break;
