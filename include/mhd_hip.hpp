@@ -15,6 +15,7 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "mhd_hip.h"
 
@@ -112,6 +113,52 @@ public:
     }
     void set_ch(double c) { check(mhd_set_ch(run_, c)); }
     void synchronize() { check(mhd_sync(run_)); }
+
+    /* initialise_grid + upload_from_aos (init.cpp:34-43) computed on the device, for the
+     * cases that have a device state (mhd_run_init_state). */
+    void init_state_device(int case_id, const mhd_ic_params* params = nullptr) {
+        check(mhd_run_init_state(run_, case_id, params));
+    }
+    /* write_all_fields (main_cpu.cpp:66-91) of the current state. */
+    void write_fields_csv(const std::string& dir, const std::string& prefix) {
+        check(mhd_write_fields_csv(run_, dir.c_str(), prefix.c_str()));
+    }
+    /* Exact dump / restore of the state and c_h. */
+    void write_state_bin(const std::string& path, double t, long long step) {
+        check(mhd_write_state_bin(run_, path.c_str(), t, step));
+    }
+    std::pair<double, long long> read_state_bin(const std::string& path) {
+        double t = 0.0;
+        long long step = 0;
+        check(mhd_read_state_bin(run_, path.c_str(), &t, &step));
+        return {t, step};
+    }
+    /* The time loop of main_cpu.cpp:176-276 for case `case_id` (name for the files),
+     * with its snapshot schedule; out_dir empty = no files. */
+    mhd_drive_result drive(int case_id, const std::string& case_name, double t_end, const std::string& out_dir,
+                           int n_scale = 1, double print_interval = 0.1, bool verbose = true) {
+        int n = 0;
+        check(mhd_case_snapshots(case_id, n_scale, nullptr, nullptr, 0, 0, &n));
+        std::vector<double> times(n > 0 ? n : 1);
+        std::vector<char> tags(16 * (n > 0 ? n : 1));
+        if (n > 0) check(mhd_case_snapshots(case_id, n_scale, times.data(), tags.data(), 16, n, &n));
+        std::vector<const char*> tp(n > 0 ? n : 1);
+        for (int k = 0; k < n; ++k) tp[k] = tags.data() + 16 * k;
+        mhd_drive_opts o{};
+        o.case_name = case_name.c_str();
+        o.platform = "gpu";
+        o.out_dir = out_dir.empty() ? nullptr : out_dir.c_str();
+        o.n_scale = n_scale;
+        o.t_end = t_end;
+        o.n_snapshots = n;
+        o.snapshot_times = times.data();
+        o.snapshot_tags = tp.data();
+        o.print_interval = print_interval;
+        o.verbose = verbose ? 1 : 0;
+        mhd_drive_result r{};
+        check(mhd_drive(run_, &o, &r));
+        return r;
+    }
 
     mhd_run* handle() { return run_; }
 

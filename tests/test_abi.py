@@ -120,7 +120,10 @@ int main() {
         mhd_hip::Run run(32, 32, 0.0, 1.0, 0.0, 1.0, 5.0 / 3.0, 0.8, mhd_hip::RiemannSolver::HLLD, bc);
         double t = 0.0;
         for (int s = 0; s < 5; ++s) { const double dt = run.compute_dt(0.8); run.advance_second_order(dt); t += dt; }
-        std::printf("ran %.17g\n", t);
+        mhd_hip::Run bw(800, 4, 0.0, 1.0, 0.0, 4.0 / 800.0, 2.0, 0.4, mhd_hip::RiemannSolver::HLLD, bc);
+        bw.init_state_device(MHD_CASE_BRIO_WU);
+        const mhd_drive_result r = bw.drive(MHD_CASE_BRIO_WU, "brio_wu", 0.1, "", 1, 0.1, false);
+        std::printf("ran %.17g %lld\n", t, r.steps);
     } catch (const std::runtime_error& e) {
         std::printf("error: %s\n", e.what());
     }
