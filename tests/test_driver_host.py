@@ -153,3 +153,27 @@ def test_compare_dirs_pass_rule_and_exit_codes(tmp_path):
     assert compare.compare_dirs(tmp_path / "e", tmp_path / "b", out=io.StringIO()) == 1  # shape mismatch
     assert compare.compare_dirs(tmp_path / "nope", tmp_path / "b", out=io.StringIO()) == 2
     assert compare.main([str(tmp_path / "a"), str(tmp_path / "b")]) == 0
+
+
+def test_writer_and_reader_errors(tmp_path):
+    """A blocked output directory is an I/O error (no partial files); foreign or
+    truncated dumps are refused."""
+    from mhd_gpu_acc_amd._lib import MHDError
+    blocker = tmp_path / "taken"
+    blocker.write_text("a file where the directory should be")
+    U = np.ones((8, 9, 9))
+    with pytest.raises(MHDError, match="create_directories"):
+        write_fields_csv_host(U, 1.4, str(blocker / "sub"), "x")
+    out = tmp_path / "ok"
+    write_fields_csv_host(U, 1.4, str(out), "x")
+    assert sorted(p.name for p in out.iterdir()) == sorted(f"x_{f}.csv" for f in FIELDS)
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"NOTASTATE" + b"\0" * 200)
+    with pytest.raises(ValueError, match="not an MHD state dump"):
+        load_bin(str(bad))
+    head = b"MHDSTATE" + struct.pack("<IIiiii", 1, BIN_HEADER_BYTES, 4, 4, 2, 9) + \
+        struct.pack("<dqdd", 0.0, 0, 1.4, 0.0) + struct.pack("<I", 1)
+    head += b"\0" * (BIN_HEADER_BYTES - len(head))
+    bad.write_bytes(head + b"\0" * 64)  # 8 of 9*8*8 doubles
+    with pytest.raises(ValueError, match="short file"):
+        load_bin(str(bad))
