@@ -284,7 +284,8 @@ typedef struct {
  * the end the [TIMING] line and, without a schedule, the final state tagged
  * t<round(100 t_end)>.  One deliberate difference: the snapshot index advances without
  * output too, so a run with out_dir = NULL reaches t_end (the reference stops there with
- * dt = 0).  Returns MHD_OK with result->status, or an error (compute_dt's, I/O). */
+ * dt = 0).  Returns MHD_OK with result->status, or an error (compute_dt's, I/O); the
+ * result holds the t and step reached either way. */
 int mhd_drive(mhd_run* run, const mhd_drive_opts* opts, mhd_drive_result* result);
 
 /* ---- measurement (no counterpart in the reference, whose only timer is the

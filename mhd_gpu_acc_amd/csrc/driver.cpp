@@ -291,11 +291,12 @@ int mhd_drive(mhd_run* r, const mhd_drive_opts* o, mhd_drive_result* res) {
         out << "\n  step       t         dt        max|B|\n";
         out << "  ----  ----------  ----------  ----------\n";
     }
+    int err = MHD_OK;  // an error stops the loop; result still reports the t and step reached
     while (t < o->t_end) {
         double t_next = o->t_end;
         if (has_snaps && snap < o->n_snapshots) t_next = std::min(t_next, o->snapshot_times[snap]);
         double dt_raw = 0.0;
-        if (int rc = mhd_compute_dt(r, &dt_raw)) return rc;  // the reference throws (solver_cpu.cpp:304-306)
+        if ((err = mhd_compute_dt(r, &dt_raw))) break;  // the reference throws (solver_cpu.cpp:304-306)
         const double dt = std::min(dt_raw, t_next - t);
         res->last_dt = dt;
         if (!std::isfinite(dt) || dt <= 0.0) {
@@ -313,13 +314,13 @@ int mhd_drive(mhd_run* r, const mhd_drive_opts* o, mhd_drive_result* res) {
             status = MHD_DRIVE_DT_FLOOR;
             break;
         }
-        if (int rc = mhd_advance(r, dt)) return rc;
+        if ((err = mhd_advance(r, dt))) break;
         t += dt;
         step += 1;
 
         if (o->print_interval > 0.0 && (t >= t_print_next || t >= o->t_end)) {
             double mb = 0.0;
-            if (int rc = max_b(r, &mb)) return rc;
+            if ((err = max_b(r, &mb))) break;
             if (o->verbose)
                 out << "  " << std::setw(4) << step << "  " << std::setw(10) << std::fixed << std::setprecision(5) << t
                     << "  " << std::setw(10) << std::scientific << std::setprecision(3) << dt << "  "
@@ -335,10 +336,11 @@ int mhd_drive(mhd_run* r, const mhd_drive_opts* o, mhd_drive_result* res) {
                     if (o->verbose)
                         out << "  [snap] " << o->snapshot_tags[snap] << "  t_phys=" << std::scientific
                             << std::setprecision(6) << t << " s\n";
-                    if (int rc = write(o->snapshot_tags[snap])) return rc;
+                    if ((err = write(o->snapshot_tags[snap]))) break;
                 }
                 ++snap;
             }
+            if (err) break;
         }
         if (o->max_steps > 0 && step >= o->max_steps && t < o->t_end) {
             status = MHD_DRIVE_MAX_STEPS;
@@ -350,6 +352,7 @@ int mhd_drive(mhd_run* r, const mhd_drive_opts* o, mhd_drive_result* res) {
     res->steps = step;
     res->elapsed_s = elapsed;
     res->status = status;
+    if (err) return err;
     if (o->verbose) {
         const double sps = static_cast<double>(step) / elapsed;
         const double mcs = static_cast<double>(step) * r->d.nx * r->d.ny / elapsed / 1.0e6;
