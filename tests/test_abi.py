@@ -107,7 +107,8 @@ def test_no_device_is_an_error_not_a_fallback():
 def test_cpp_front_compiles_links_and_reports(tmp_path):
     """include/mhd_hip.hpp (the head/-shaped C++ front) compiles with g++, links against
     the library, and runs: without a GPU the run constructor throws the library's
-    no-device error (no CPU fallback); with one it steps."""
+    no-device error (no CPU fallback); with one it steps Brio-Wu from its device initial
+    state and drives it to t_end in the reference driver's 978 steps."""
     import subprocess
     src = tmp_path / "front.cpp"
     src.write_text(r'''
@@ -115,11 +116,12 @@ def test_cpp_front_compiles_links_and_reports(tmp_path):
 #include "mhd_hip.hpp"
 int main() {
     try {
-        mhd_hip::BoundaryConfig bc;
-        bc.left = bc.right = bc.bottom = bc.top = mhd_hip::BoundaryType::Periodic;
-        mhd_hip::Run run(32, 32, 0.0, 1.0, 0.0, 1.0, 5.0 / 3.0, 0.8, mhd_hip::RiemannSolver::HLLD, bc);
+        mhd_hip::BoundaryConfig bc;  // Brio-Wu: transmissive in x, periodic in y (test_cases.cpp)
+        bc.bottom = bc.top = mhd_hip::BoundaryType::Periodic;
+        mhd_hip::Run run(800, 4, 0.0, 1.0, 0.0, 4.0 / 800.0, 2.0, 0.4, mhd_hip::RiemannSolver::HLLD, bc);
+        run.init_state_device(MHD_CASE_BRIO_WU);
         double t = 0.0;
-        for (int s = 0; s < 5; ++s) { const double dt = run.compute_dt(0.8); run.advance_second_order(dt); t += dt; }
+        for (int s = 0; s < 5; ++s) { const double dt = run.compute_dt(0.4); run.advance_second_order(dt); t += dt; }
         mhd_hip::Run bw(800, 4, 0.0, 1.0, 0.0, 4.0 / 800.0, 2.0, 0.4, mhd_hip::RiemannSolver::HLLD, bc);
         bw.init_state_device(MHD_CASE_BRIO_WU);
         const mhd_drive_result r = bw.drive(MHD_CASE_BRIO_WU, "brio_wu", 0.1, "", 1, 0.1, false);
@@ -138,3 +140,5 @@ int main() {
                     "-Wl,-rpath,/opt/rocm/lib"], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120).stdout
     assert out.startswith("ran ") or "no HIP device" in out, out
+    if out.startswith("ran "):
+        assert out.split()[2] == "978", out  # the reference driver's Brio-Wu step count
