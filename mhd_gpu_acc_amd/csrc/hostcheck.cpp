@@ -231,4 +231,19 @@ int hc_adjoint_step(const double* Uold, const double* Utmp, int nx, int ny, doub
     return 1;
 }
 
+// compute_dt candidates of n cells (conserved, reference order, 9 per cell): the
+// kernels' cfl_candidate, which returns c_h for cells its speed test clears, and
+// the exact evaluation of every cell.  hit[i] = 1 where the test cleared cell i.
+void hc_cfl_candidates(const double* U, int n, double gamma, double ch, double* fast, double* exact, int* hit) {
+    const Phys ph = phys(gamma, ch);
+    for (int i = 0; i < n; ++i) {
+        const Cons u = rot(U + 9 * (size_t)i, true);
+        fast[i] = cfl_candidate(u, ph);
+        const Prim w = to_prim(u, ph);
+        const bool valid = finite(w.r) && finite(w.p) && w.r > 0.0 && w.p > 0.0;
+        exact[i] = valid ? cfl_candidate_exact(w, ph) : 0.0;
+        hit[i] = valid && speeds_below_ch(w, ph);
+    }
+}
+
 }  // extern "C"

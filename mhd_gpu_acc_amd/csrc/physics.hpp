@@ -45,8 +45,7 @@ struct Phys {
     double half_over_ch;  // 0.5 / ch (riemann.hpp:557)
 };
 
-MHD_DI bool finite(double x) { return __builtin_isfinite(x); }
-MHD_DI double std_min(double a, double b) { return (b < a) ? b : a; }  // std::min
+MHD_DI bool finite(double x) { return __builtin_isfinite(x); }MHD_DI double std_min(double a, double b) { return (b < a) ? b : a; }  // std::min
 MHD_DI double std_max(double a, double b) { return (a < b) ? b : a; }  // std::max
 
 // physics.hpp:69-78
@@ -428,13 +427,8 @@ MHD_DI Cons update(const Cons& u, double dtd, const Cons& fp, const Cons& fm) {
     return n;
 }
 
-// Candidate of compute_dt for one cell (solver_cpu.cpp:291-300): the larger of the
-// x and y signal speeds, or 0 when the cell is skipped.  `u` is in the x frame.
-// a^2 and b^2 are the same expressions for both directions (physics.hpp:176-177,
-// 185-186) and are formed once.
-MHD_DI double cfl_candidate(const Cons& u, const Phys& ph) {
-    const Prim w = to_prim(u, ph);
-    if (!finite(w.r) || !finite(w.p) || w.r <= 0.0 || w.p <= 0.0) return 0.0;
+// The exact candidate of a valid cell (physics.hpp:175-210 for both directions).
+MHD_DI double cfl_candidate_exact(const Prim& w, const Phys& ph) {
     const double a2 = ph.gamma * w.p / w.r;
     const double b2 = w.b2 / w.r;
     const double bx2 = w.bn * w.bn / w.r;
@@ -448,6 +442,60 @@ MHD_DI double cfl_candidate(const Cons& u, const Phys& ph) {
     const double sy = fmax(fabs(w.ut) + cy, ph.ch);
     if (finite(sx) && finite(sy)) return std_max(sx, sy);
     return 0.0;
+}
+
+// Sufficient test, without division or square root, that the fast speed along
+// one direction cannot lift the reference's signal speed above c_h:
+// fl(|u_n| + c_f) <= c_h, so max_signal_speed (physics.hpp:197-210), an fmax with
+// c_h, returns c_h itself.  In exact arithmetic, with Z = c_h - |u_n| > 0,
+//   c_f^2 = (S + sqrt(S^2 - 4 a^2 b_n^2)) / 2 <= Z^2,   S = a^2 + b^2,
+// holds iff 2Z^2 >= S and Z^2 (S - Z^2) <= a^2 b_n^2; multiplied by rho^2, with
+// Y = Z^2 rho and T = gamma p + |B|^2:  2Y >= T and Y (T - Y) <= gamma p B_n^2
+// (Y >= T alone suffices: it bounds c_f by sqrt(S)).
+// Margins.  The rounded c_f exceeds the exact one by at most a relative 2.1e-8
+// (the discriminant cancels: an absolute error of 15 ulp of S^2 under a square
+// root), so Z is taken from c_h (1 - 1e-7).  (Where that cancellation happens,
+// a^2 ~ b_n^2 ~ b^2, the second condition is near equality and its own margin
+// already demands Z^2 >= c_f^2 (1 + ~2e-6).)  Each rounded product and sum of the
+// test is within a few ulp of its exact value; relative margins of 1e-12 absorb
+// that, always towards "not proven".  The caller keeps gamma p in [1e-100, 1e100],
+// T and rho <= 1e100, and y is held <= 1e100, so that every product a passing test
+// relies on is normal and finite (a y that underflows is far below T and fails),
+// except B_n^2, whose underflow error (<= 2.5e-224 after the product) the
+// absolute margin covers.  Outside those bounds the test fails and the speed is
+// evaluated exactly.
+MHD_DI bool speed_below_ch(double un, double bn, double gp, double t, double rho, double ch) {
+    constexpr double M = 1.0e-12;
+    const double z = ch * (1.0 - 1.0e-7) - fabs(un);
+    if (!(z > 0.0)) return false;
+    const double y = z * z * rho;
+    if (!(y <= 1.0e100)) return false;
+    if (y >= t * (1.0 + M)) return true;
+    if (!(2.0 * y >= t * (1.0 + M))) return false;
+    const double lhs = y * ((t - y) + M * (t + y)) * (1.0 + M);
+    const double rhs = gp * (bn * bn) * (1.0 - M) - 1.0e-220;
+    return lhs <= rhs;
+}
+
+// Both directions of a valid cell within the magnitude bounds speed_below_ch needs.
+MHD_DI bool speeds_below_ch(const Prim& w, const Phys& ph) {
+    const double gp = ph.gamma * w.p;
+    const double t = gp + w.b2;
+    return gp >= 1.0e-100 && t <= 1.0e100 && w.r <= 1.0e100 && speed_below_ch(w.un, w.bn, gp, t, w.r, ph.ch) &&
+           speed_below_ch(w.ut, w.bt, gp, t, w.r, ph.ch);
+}
+
+// Candidate of compute_dt for one cell (solver_cpu.cpp:291-300): the larger of the
+// x and y signal speeds, or 0 when the cell is skipped.  `u` is in the x frame.
+// a^2 and b^2 are the same expressions for both directions (physics.hpp:176-177,
+// 185-186) and are formed once.  Cells whose speeds are provably at most c_h
+// (speeds_below_ch) return c_h, the value the exact evaluation returns for them,
+// without the divisions and square roots; c_h ratchets, so that is most cells.
+MHD_DI double cfl_candidate(const Cons& u, const Phys& ph) {
+    const Prim w = to_prim(u, ph);
+    if (!finite(w.r) || !finite(w.p) || w.r <= 0.0 || w.p <= 0.0) return 0.0;
+    if (speeds_below_ch(w, ph)) return ph.ch;
+    return cfl_candidate_exact(w, ph);
 }
 
 }  // namespace mhd

@@ -27,6 +27,7 @@ def lib():
         L.hc_prim_to_cons_adj.argtypes = [_dp, C.c_double, _dp, _dp]
         L.hc_fast_speed_adj.argtypes = [_dp, C.c_int, C.c_double, C.c_double, _dp]
         L.hc_limiter_adj.argtypes = [C.c_double, C.c_double, C.c_int, C.c_double, _dp, _dp]
+        L.hc_cfl_candidates.argtypes = [_dp, C.c_int, C.c_double, C.c_double, _dp, _dp, C.POINTER(C.c_int)]
         L.hc_adjoint_step.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                       C.POINTER(C.c_int), _dp]
@@ -96,3 +97,14 @@ def adjoint_step(Uold, Utmp, case, dt, ch, solver, limiter, lam, cr=0.18, bc=Non
     if rc != 0:
         raise ValueError("unsupported solver, limiter or boundary for the adjoint")
     return out
+
+
+def cfl_candidates(U, gamma, ch):
+    """The kernels' compute_dt candidate of each cell (rows of U: conserved, reference
+    order), the exact candidate, and whether the speed test cleared the cell."""
+    U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1, 9)
+    n = U.shape[0]
+    fast, exact = np.zeros(n), np.zeros(n)
+    hit = np.zeros(n, dtype=np.int32)
+    lib().hc_cfl_candidates(_p(U), n, gamma, ch, _p(fast), _p(exact), hit.ctypes.data_as(C.POINTER(C.c_int)))
+    return fast, exact, hit.astype(bool)
