@@ -159,6 +159,25 @@ def main():
                                "sample": f"first {k} steps of the same {n}^2 run ({secs:.1f} s), "
                                          f"C++/OpenMP restatement of the reference CPU path"}
         out["parity"] = {"dt_sequence_bitwise_vs_cpu": same, "steps_compared": kk}
+        if oracle.ref_available():
+            # The reference's own CPU solver (compiled from its sources into oracle/_ref) on
+            # the same grid.  It has no van Leer limiter, so this sample uses its minmod
+            # slopes; the port runs the same minmod sample for comparison.
+            kr = int(max(1, min(40, round(0.5 * args.cpu_seconds / max(t_one, 1e-9)))))
+            rref, sref = oracle.ref_run_loop_timed(U0, case, solver, kr)
+            rprt, sprt = oracle.run_loop_timed(U0, case, solver, 0, kr)
+            kq = min(rref["nsteps"], rprt["nsteps"])
+            out["cpu_reference"] = {
+                "value": kr * cells / sref, "unit": "cell-updates/s", "cores": oracle.num_threads(),
+                "kind": "reference",
+                "sample": f"first {kr} steps of the same {n}^2 Orszag-Tang run with {args.solver.upper()} and "
+                          f"minmod slopes (the reference has no van Leer limiter), {sref:.1f} s: the reference "
+                          f"CPU solver compiled from its own sources",
+                "port_same_sample": kr * cells / sprt,
+                "dt_sequence_bitwise_vs_port": bool(np.array_equal(rref["dts"][:kq].view(np.uint64),
+                                                                   rprt["dts"][:kq].view(np.uint64))),
+                "state_bitwise_vs_port": bool(np.array_equal(rref["U"].view(np.uint64),
+                                                             rprt["U"].view(np.uint64)))}
     print(json.dumps(out), flush=True)
 
 

@@ -126,8 +126,8 @@ MHD_DI FastSpeed fast_speed_fwd(const Prim& w, const Phys& ph) {
     f.bn2 = w.bn * w.bn / w.r;
     f.s = f.a2 + f.b2;
     f.disc = f.s * f.s - 4.0 * f.a2 * f.bn2;
-    f.q = sqrt(f.disc > 0.0 ? f.disc : 0.0);
-    f.c = sqrt(0.5 * (f.s + f.q));
+    f.q = sqrt_cr(f.disc > 0.0 ? f.disc : 0.0);
+    f.c = sqrt_cr(0.5 * (f.s + f.q));
     return f;
 }
 
@@ -391,8 +391,8 @@ MHD_DI int hlld_adj(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& 
     }
     const double a2 = sm - vl.un;
     const double ptstar = ptl + vl.r * a1 * a2;
-    const double sql = sqrt(rls);
-    const double sqr = sqrt(rrs);
+    const double sql = sqrt_cr(rls);
+    const double sqr = sqrt_cr(rrs);
     const double ql = fabs(vl.bn) / sql;
     const double qr = fabs(vr.bn) / sqr;
     const double sls = sm - ql;

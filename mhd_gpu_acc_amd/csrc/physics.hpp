@@ -102,13 +102,38 @@ MHD_DI Cons flux(const Cons& u, const Prim& w, const Phys& ph) {
     return f;
 }
 
+// Square root with the bits of sqrt().  On the device sqrt is lowered to rsq
+// followed by two Newton-Raphson corrections, wrapped in a range scaling (taken for
+// x below 2^-767) and a fixup that passes 0 and +inf through; for x in
+// [2^-767, DBL_MAX] the wrapping is the identity.  A wave whose active lanes all
+// hold such an x runs the core sequence alone, operation for operation as the
+// lowering orders it (10 instructions instead of 18); any other wave calls sqrt.
+MHD_DI double sqrt_cr(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const bool core = x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023;
+    if (__builtin_amdgcn_ballot_w64(!core) == 0) {
+        double h = __builtin_amdgcn_rsq(x);
+        double g = x * h;
+        h = h * 0.5;
+        const double e = __builtin_fma(-h, g, 0.5);
+        g = __builtin_fma(g, e, g);
+        h = __builtin_fma(h, e, h);
+        double d = __builtin_fma(-g, g, x);
+        g = __builtin_fma(d, h, g);
+        d = __builtin_fma(-g, g, x);
+        return __builtin_fma(d, h, g);
+    }
+#endif
+    return sqrt(x);
+}
+
 // physics.hpp:175-190 (fast magnetosonic speed along the normal)
 MHD_DI double fast_speed(const Prim& w, const Phys& ph) {
     const double a2 = ph.gamma * w.p / w.r;
     const double b2 = w.b2 / w.r;
     const double bn2 = w.bn * w.bn / w.r;
     const double disc = (a2 + b2) * (a2 + b2) - 4.0 * a2 * bn2;
-    return sqrt(0.5 * (a2 + b2 + sqrt(disc > 0.0 ? disc : 0.0)));
+    return sqrt_cr(0.5 * (a2 + b2 + sqrt_cr(disc > 0.0 ? disc : 0.0)));
 }
 
 // riemann.hpp:54-57
@@ -261,8 +286,8 @@ MHD_DI bool hlld(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr,
     if (sl >= 0.0) { f = flux(ul, vl, ph); return true; }
     if (sr <= 0.0) { f = flux(ur, vr, ph); return true; }
     const double ptstar = ptl + vl.r * (sl - vl.un) * (sm - vl.un);
-    const double sql = sqrt(rls);
-    const double sqr = sqrt(rrs);
+    const double sql = sqrt_cr(rls);
+    const double sqr = sqrt_cr(rrs);
     const double sls = sm - fabs(vl.bn) / sql;
     const double srs = sm + fabs(vr.bn) / sqr;
     const double psim = 0.5 * (vl.psi + vr.psi) - ph.half_ch * (vr.bn - vl.bn);
@@ -436,8 +461,8 @@ MHD_DI double cfl_candidate_exact(const Prim& w, const Phys& ph) {
     const double s = (a2 + b2) * (a2 + b2);
     const double dx = s - 4.0 * a2 * bx2;
     const double dy = s - 4.0 * a2 * by2;
-    const double cx = sqrt(0.5 * (a2 + b2 + sqrt(dx > 0.0 ? dx : 0.0)));
-    const double cy = sqrt(0.5 * (a2 + b2 + sqrt(dy > 0.0 ? dy : 0.0)));
+    const double cx = sqrt_cr(0.5 * (a2 + b2 + sqrt_cr(dx > 0.0 ? dx : 0.0)));
+    const double cy = sqrt_cr(0.5 * (a2 + b2 + sqrt_cr(dy > 0.0 ? dy : 0.0)));
     const double sx = fmax(fabs(w.un) + cx, ph.ch);
     const double sy = fmax(fabs(w.ut) + cy, ph.ch);
     if (finite(sx) && finite(sy)) return std_max(sx, sy);

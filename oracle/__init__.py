@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import time
 from dataclasses import dataclass
 
 import numpy as np
@@ -238,6 +239,39 @@ def run_loop_timed(U: np.ndarray, case: Case, solver: int, limiter: int, max_ste
                                     C.byref(ch), ptr(dts), ptr(chs), C.byref(n), C.byref(t), C.byref(sec))
     k = n.value
     return dict(U=U, dts=dts[:k].copy(), chs=chs[:k].copy(), nsteps=k, t=t.value, code=code), sec.value
+
+
+def ref_run_loop_timed(U: np.ndarray, case: Case, solver: int, max_steps: int,
+                       t_end: float | None = None, cr: float = 0.18):
+    """The compiled reference's own driver loop (ref_shim.cpp ref_run_loop: compute_dt +
+    advance_second_order of src/cpu/solver_cpu.cpp, minmod slopes, its only limiter)
+    from state U; returns (dict(U, dts, chs, nsteps, t, code), wall seconds of the
+    loop alone)."""
+    lib = ref()
+    lib.ref_set_gamma(case.gamma)
+    lib.ref_set_ch(0.0)
+    lib.ref_set_cr(cr)
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    h = [lib.ref_grid_new(case.nx, case.ny, case.ng, case.x_min, case.x_max, case.y_min, case.y_max)
+         for _ in range(3)]
+    try:
+        for x in h:
+            lib.ref_grid_set(x, ptr(U))
+        dts = np.zeros(max(max_steps, 1))
+        chs = np.zeros(max(max_steps, 1))
+        ns = C.c_int()
+        t = C.c_double()
+        t0 = time.perf_counter()
+        code = lib.ref_run_loop(h[0], h[1], h[2], case.cfl, case.t_end if t_end is None else t_end, solver,
+                                ibuf(case.bc), max_steps, ptr(dts), ptr(chs), C.byref(ns), C.byref(t))
+        sec = time.perf_counter() - t0
+        Ur = np.zeros_like(U)
+        lib.ref_grid_get(h[0], ptr(Ur))
+    finally:
+        for x in h:
+            lib.ref_grid_free(x)
+    k = ns.value
+    return dict(U=Ur, dts=dts[:k].copy(), chs=chs[:k].copy(), nsteps=k, t=t.value, code=code), sec
 
 
 def num_threads() -> int:
