@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form measurement")
     ap.add_argument("--batch-steps", type=int, default=40)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other plain configurations (all solvers at 4096^2 and 1024^2, shock-cloud, Brio-Wu)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="before the timed steps, copy BYTES between two device buffers once (a kernel "
                          "of known traffic for calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -144,6 +146,8 @@ def main():
         out["batched"] = bench_batch(args.batch_steps)
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint()
+    if not args.no_configs:
+        out["configs"] = bench_configs()
 
     if not args.no_cpu:
         import oracle
@@ -179,6 +183,71 @@ def main():
                 "state_bitwise_vs_port": bool(np.array_equal(rref["U"].view(np.uint64),
                                                              rprt["U"].view(np.uint64)))}
     print(json.dumps(out), flush=True)
+
+
+_IC_CACHE = {}
+
+
+def _time_plain(case_name, nx, ny, solver, limiter, warmup, steps, t_end=None, params=None):
+    """Plain step of one configuration: warm-up, then `steps` timed steps (or the run to
+    t_end when steps is None), with the sweep kernels timed by HIP events.  A run the
+    scheme cannot finish (the reference's blow-up stop, dt below 1e-8 t_end) reports
+    its stop code and the time reached, as the reference driver does."""
+    import ctypes as C
+    import time
+
+    import mhd_gpu_acc_amd as m
+
+    cfg = m.get_n_case_config(case_name, 1)
+    cfg.nx, cfg.ny = nx, ny
+    if case_name == "brio_wu_transmissive":
+        cfg.cfl = 0.8
+    key = (case_name, nx, ny)
+    if key not in _IC_CACHE:
+        _IC_CACHE.clear()
+        _IC_CACHE[key] = m.initial_state(m.parse_case_id(case_name), cfg, params)
+    U0 = _IC_CACHE[key]
+    L = m.lib()
+    with m.Run(cfg, SOLVERS[solver], LIMITERS[limiter]) as run:
+        run.set_state(U0)
+        if warmup:
+            run.step_to(cfg.t_end, warmup)
+        L.mhd_profile(run.handle, 1)
+        run.synchronize()
+        t0 = time.perf_counter()
+        r = run.step_to(cfg.t_end if t_end is None else t_end, steps if steps is not None else 10 ** 7)
+        run.synchronize()
+        sec = time.perf_counter() - t0
+        ms4 = (C.c_double * 4)()
+        nst = C.c_int()
+        L.mhd_profile_read(run.handle, ms4, C.byref(nst))
+        L.mhd_profile(run.handle, 0)
+    k = max(nst.value, 1)
+    ysweep = ms4[2] / k
+    return {"workload": f"{case_name} {nx}x{ny}, {solver.upper()}, {limiter}", "steps": r["steps"],
+            "t": r["t"], "stop_code": r["code"], "value": r["steps"] * nx * ny / sec, "unit": "cell-updates/s",
+            "ms_per_step": 1e3 * sec / max(r["steps"], 1), "xsweep_ms": ms4[0] / k, "ysweep_ms": ysweep,
+            "ysweep_hbm_frac": BYTES_PER_CELL_SWEEP * nx * ny / (ysweep * 1e-3) / HBM_PEAK}
+
+
+def bench_configs():
+    """The other plain configurations: Orszag-Tang at 4096^2 and 1024^2 with each Riemann
+    solver (van Leer, 5 warm-up + 100 timed steps), the 1024^2 HLLD run to t = 0.5, the
+    seeded shock-cloud 2048x1024 (HLLD, 200 steps) and Brio-Wu 800x4 (HLLD, minmod,
+    CFL 0.8, to t = 0.1)."""
+    import mhd_gpu_acc_amd as m
+
+    res = []
+    for n in (4096, 1024):
+        for s in SOLVERS:
+            res.append(_time_plain("orszag_tang_unit", n, n, s, "vanleer", 5, 100))
+    res.append(_time_plain("orszag_tang_unit", 1024, 1024, "hlld", "vanleer", 0, None, t_end=0.5))
+    p = m.ICParams()
+    p.seed = 20261203
+    res.append(_time_plain("shock_cloud", 2048, 1024, "hlld", "vanleer", 5, 200, params=p))
+    res.append(_time_plain("brio_wu_transmissive", 800, 4, "hlld", "minmod", 0, None))
+    _IC_CACHE.clear()
+    return res
 
 
 def bench_batch(steps: int, seed: int = 20261203):
