@@ -67,18 +67,39 @@ __global__ __launch_bounds__(WG) void batch_xsweep_kernel(const BatchRun* __rest
         for (int r = t; r < nruns; r += WG) acc[r] = 0ull;  // CFL maxima of this step's y sweep
         if (t == 0) *queue = 0u;                             // y-sweep work counter
     }
-    for (int q = blockIdx.x; q < ntiles; q += gridDim.x) {
-        const int2 tl = tiles[q];
+    int q = blockIdx.x;
+    if (q >= ntiles) return;
+    // The cells of tile q + gridDim.x are loaded before tile q is computed, so their
+    // load latency overlaps the arithmetic (as the plain x sweep's segments).
+    const auto load_tile = [&](int2 tl) {
+        const BatchRun& R = runs[tl.x];
+        const BatchStep& P = steps[tl.x];
+        const Geom g = geom_of(R);
+        const Planes<true, GlobalT<const double>> src(as_global<const double>(R.U[P.cur]), g.ps);
+        const int k = NG * g.tnx - 2 + tl.y * (WG - 4) + t;
+        return src.load(min(k, g.tnx * g.tny - 1));
+    };
+    int2 tl = tiles[q];
+    Cons un = load_tile(tl);
+    for (;;) {
+        const int qn = q + (int)gridDim.x;
+        const Cons u = un;
+        int2 tln = tl;
+        if (qn < ntiles) {
+            tln = tiles[qn];
+            un = load_tile(tln);
+        }
         const BatchRun& R = runs[tl.x];
         const BatchStep& P = steps[tl.x];
         const Geom g = geom_of(R);
         const Phys ph = P.ph;
-        const Planes<true, GlobalT<const double>> src(as_global<const double>(R.U[P.cur]), g.ps);
         const Planes<true, GlobalT<double>> dst(as_global(R.U[1]), g.ps);
         const int k = NG * g.tnx - 2 + tl.y * (WG - 4) + t;
-        const Cons u = src.load(min(k, g.tnx * g.tny - 1));
         xsweep_segment<SOLVER, LIMITER, WG>(u, k, t, dst, g, ph, P.dtdx, P.hdtdx, L);
+        if (qn >= ntiles) break;
         lds_barrier();  // LDS is rewritten by the next tile
+        q = qn;
+        tl = tln;
     }
 }
 

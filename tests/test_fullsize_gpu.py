@@ -5,7 +5,8 @@ solver) or the oracle's (van Leer runs, which the reference lacks) bit for bit.
 
 - Orszag-Tang 4096^2, HLLD, 100 steps (the measured workload);
 - Orszag-Tang 1024^2 to t = 0.5, all four Riemann solvers x both limiters (the
-  accuracy runs);
+  accuracy runs; at CFL 0.8 the HLLC and HLLD runs stop at the driver's blow-up guard,
+  dt < 1e-8 t_end, and must stop at the same step with the same state);
 - the shock-cloud 2048x1024, HLLD, 200 steps (seeded density noise);
 - Brio-Wu 800x4, HLLD, minmod, CFL 0.8, transmissive, to t = 0.1.
 """
