@@ -24,7 +24,7 @@ import tempfile
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 BIN = os.path.join(ROOT, "oracle", "_ref", "main_cpu")
 OUT = os.path.join(ROOT, "tests", "golden", "driver_reference.json")
-CASES = ("brio_wu", "orszag_tang", "shock_bubble")
+CASES = ("brio_wu", "orszag_tang", "shock_bubble", "kelvin_helmholtz")
 PROGRESS = re.compile(r"^\s+\d+\s+\d+\.\d{5}\s+\S+\s+\S+$")
 
 

@@ -1,6 +1,7 @@
 """The run driver on the GPU against the reference's own CPU driver
 (tests/golden/driver_reference.json, from oracle/_ref/main_cpu): for Brio-Wu,
-Orszag-Tang (snapshot schedule) and shock-bubble (nine snapshots), every snapshot
+Orszag-Tang (snapshot schedule), shock-bubble (nine snapshots) and Kelvin-Helmholtz
+(4 695 steps), every snapshot
 CSV is byte for byte the reference's, the progress lines (step, t, dt, max|B|) and
 [snap] lines are the reference's verbatim, and the step count is the same.  A run
 restarted from a state dump ends in the same files as the uninterrupted run; a
@@ -45,7 +46,7 @@ def _check_files(out_dir, ref, only=None):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["brio_wu", "orszag_tang", "shock_bubble"])
+@pytest.mark.parametrize("case", ["brio_wu", "orszag_tang", "shock_bubble", "kelvin_helmholtz"])
 def test_driver_matches_reference_driver(case, tmp_path):
     ref = json.load(open(GOLDEN))[case]
     say(f"[gpu] driver {case} hlld minmod n=1 ({ref['steps']} steps)")
