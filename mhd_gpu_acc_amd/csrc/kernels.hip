@@ -40,18 +40,12 @@ __global__ __launch_bounds__(WG) void xsweep_kernel(const double* __restrict__ U
     int s = blockIdx.x;
     if (s >= nseg) return;
     int k = kb + s * (WG - 4);
-#if MHD_X_PREFETCH
     Cons un = src.load(min(k, last));  // clamp: tail lanes read valid memory
-#endif
     for (;;) {
         const int sn = s + gridDim.x;
         const int kn = kb + sn * (WG - 4);
-#if MHD_X_PREFETCH
         const Cons u = un;  // the segment loaded one iteration ago
         if (sn < nseg) un = src.load(min(kn, last));
-#else
-        const Cons u = src.load(min(k, last));
-#endif
         xsweep_segment<SOLVER, LIMITER, WG>(u, k, t, dst, g, ph, dtd, hdtd, L);
         if (sn >= nseg) break;
         lds_barrier();  // sw is rewritten by the next segment

@@ -88,9 +88,6 @@ template <int WG, int NF = 9> struct Lds9 {
 // rows; lanes 2..WG-3 that land on an interior column write Utmp.  A block walks
 // segments blockIdx.x, +gridDim.x, ... and loads the next segment's cells before
 // computing the current one, so global-load latency overlaps the arithmetic.
-#ifndef MHD_X_PREFETCH
-#define MHD_X_PREFETCH 1
-#endif
 
 template <int WG> struct XLds {
     Lds9<WG> sw;      // primitives, later face fluxes
@@ -159,14 +156,10 @@ MHD_DI void wave_max_to(unsigned long long* acc, double m) {
     atomicMax(acc, (unsigned long long)__double_as_longlong(m));
 }
 
-// Knobs (compile time): MHD_Y_WAVES_PER_EU asks the compiler for that many
-// resident waves per SIMD; MHD_Y_CARRY_PRIM keeps the primitive of the upper
-// interface state across rows instead of recomputing it.
+// Knob (compile time): MHD_Y_WAVES_PER_EU asks the compiler for that many resident
+// waves per SIMD (0: its own choice, 256 VGPRs and 2 waves for HLLD).
 #ifndef MHD_Y_WAVES_PER_EU
 #define MHD_Y_WAVES_PER_EU 0
-#endif
-#ifndef MHD_Y_CARRY_PRIM
-#define MHD_Y_CARRY_PRIM 1
 #endif
 
 #if MHD_Y_WAVES_PER_EU > 0
@@ -210,9 +203,7 @@ MHD_DI double ysweep_column(const double* __restrict__ Ut, double* __restrict__ 
     Prim w0 = to_prim(src.load((j0 - 2) * g.tnx + ic), ph);
     Prim w1 = to_prim(src.load((j0 - 1) * g.tnx + ic), ph);
     Cons hu;     // upper interface state of the previous cell
-#if MHD_Y_CARRY_PRIM
     Prim hw;     // and its primitive
-#endif
     Cons fprev;  // face flux below the cell being updated
     int slot = 0;  // ring slot of row r; row r-2 is in slot (slot + 1) % 3
     for (int r = j0; r <= j1 + 1; ++r) {
@@ -222,11 +213,7 @@ MHD_DI double ysweep_column(const double* __restrict__ Ut, double* __restrict__ 
         Face lo, hi;
         reconstruct<LIMITER>(w0, w1, w2, hdtd, ph, lo, hi);  // cell r-1
         if (r >= j0 + 1) {
-#if MHD_Y_CARRY_PRIM
             const Cons f = riemann<SOLVER, false>(hu, hw, lo.u, lo.w, ph);  // face (r-2 | r-1)
-#else
-            const Cons f = riemann<SOLVER, false>(hu, to_prim(hu, ph), lo.u, lo.w, ph);
-#endif
             if (r >= j0 + 2) {
                 const int j = r - 2;
                 const int k = j * g.tnx + ic;
@@ -245,9 +232,7 @@ MHD_DI double ysweep_column(const double* __restrict__ Ut, double* __restrict__ 
             fprev = f;
         }
         hu = hi.u;
-#if MHD_Y_CARRY_PRIM
         hw = hi.w;
-#endif
         w0 = w1;
         w1 = w2;
         slot = slot == 2 ? 0 : slot + 1;
