@@ -278,7 +278,7 @@ def test_no_signal_is_reported():
 
 @pytest.mark.parametrize("n", [1024])
 def test_full_size_orszag_tang_vs_oracle(n):
-    """BASELINE size class: OT 1024^2, HLLD van Leer, 3 steps, bitwise vs the oracle."""
+    """The measured size class: OT 1024^2, HLLD van Leer, 3 steps, bitwise vs the oracle."""
     cfg = m.get_n_case_config("orszag_tang_unit", 1)
     cfg.nx = cfg.ny = n
     U0 = m.initial_state(4, cfg)
