@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
                                                            const int4* __restrict__ tiles, int ntiles,
                                                            unsigned int* __restrict__ queue,
                                                            unsigned long long* __restrict__ acc) {
-    __shared__ double ring[27 * 256];
+    __shared__ double ring[YRING_DOUBLES * 256];
     const int lane = threadIdx.x & 63;
     const YRing yr{ring, (int)threadIdx.x, (int)blockDim.x};
     for (;;) {

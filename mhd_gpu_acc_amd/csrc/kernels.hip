@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) MHD_Y_ATTR void ysweep_kernel(const double* __
                                                                 double dtd, double hdtd, double damp, int seg,
                                                                 double* __restrict__ frame,
                                                                 unsigned long long* __restrict__ acc) {
-    __shared__ double ring[27 * 256];
+    __shared__ double ring[YRING_DOUBLES * 256];
     const int i = NG + blockIdx.x * blockDim.x + threadIdx.x;
     const int j0 = NG + blockIdx.y * seg;
     const int j1 = min(j0 + seg, NG + g.ny);

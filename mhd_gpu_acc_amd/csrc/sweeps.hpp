@@ -168,14 +168,27 @@ MHD_DI void wave_max_to(unsigned long long* acc, double m) {
 #define MHD_Y_ATTR
 #endif
 
-// The raw conserved states of the last three rows a lane loaded, kept in LDS
-// (field-major, lane-consecutive, so accesses are conflict-free): the update of row
-// r-2 reads them back instead of reloading global memory, which would expose a
-// memory latency per row and double the sweep's reads.  Each lane touches only its
-// own slots, so no barrier is needed.
+// Per-lane LDS of the column march (field-major, lane-consecutive, so accesses are
+// conflict-free; each lane touches only its own slots, so no barrier is needed):
+//   - the raw conserved states of the last three rows the lane loaded: the update of
+//     row r-2 reads them back instead of reloading global memory, which would expose
+//     a memory latency per row and double the sweep's reads;
+//   - the primitive of the row above the cell being reconstructed (the slope's lower
+//     neighbour), which is not needed while the face flux is formed: keeping it here
+//     instead of in registers leaves room for the next row's load to be in flight.
+constexpr int YRING_DOUBLES = 36;  // per lane: 3 rows + 1 primitive, 9 doubles each
 struct YRing {
-    double* p;   // 27 * n doubles of LDS
+    double* p;   // YRING_DOUBLES * n doubles of LDS
     int t, n;    // this lane's index and the lane count of the block
+    MHD_DI void putw(const Prim& w) const {
+        double* q = p + 27 * n + t;
+        q[0] = w.r; q[n] = w.un; q[2 * n] = w.ut; q[3 * n] = w.uz; q[4 * n] = w.bn;
+        q[5 * n] = w.bt; q[6 * n] = w.bz; q[7 * n] = w.p; q[8 * n] = w.psi;
+    }
+    MHD_DI Prim getw() const {
+        const double* q = p + 27 * n + t;
+        return Prim{q[0], q[n], q[2 * n], q[3 * n], q[4 * n], q[5 * n], q[6 * n], q[7 * n], q[8 * n], 0.0};
+    }
     MHD_DI void put(int slot, const Cons& u) const {
         double* q = p + slot * 9 * n + t;
         q[0] = u.r; q[n] = u.mn; q[2 * n] = u.mt; q[3 * n] = u.mz; q[4 * n] = u.bn;
@@ -200,20 +213,26 @@ MHD_DI double ysweep_column(const double* __restrict__ Ut, double* __restrict__ 
     const int ic = active ? i : NG;  // idle lanes shadow column NG (no stores)
     const bool damping = !(damp >= 1.0);  // solver_cpu.cpp:265 skips only when factor >= 1
     double cand = 0.0;
-    Prim w0 = to_prim(src.load((j0 - 2) * g.tnx + ic), ph);
+    ring.putw(to_prim(src.load((j0 - 2) * g.tnx + ic), ph));
     Prim w1 = to_prim(src.load((j0 - 1) * g.tnx + ic), ph);
     Cons hu;     // upper interface state of the previous cell
     Prim hw;     // and its primitive
     Cons fprev;  // face flux below the cell being updated
     int slot = 0;  // ring slot of row r; row r-2 is in slot (slot + 1) % 3
+    // Row r+1 is loaded during iteration r, once the face flux is formed: the update,
+    // its stores and the CFL candidate then run while the load is in flight (the last
+    // iteration loads row j1+1 again).
+    Cons ur = src.load(j0 * g.tnx + ic);
     for (int r = j0; r <= j1 + 1; ++r) {
-        const Cons ur = src.load(r * g.tnx + ic);
         ring.put(slot, ur);
         const Prim w2 = to_prim(ur, ph);
         Face lo, hi;
-        reconstruct<LIMITER>(w0, w1, w2, hdtd, ph, lo, hi);  // cell r-1
+        reconstruct<LIMITER>(ring.getw(), w1, w2, hdtd, ph, lo, hi);  // cell r-1
+        ring.putw(w1);
+        if (r < j0 + 1) ur = src.load((r + 1) * g.tnx + ic);
         if (r >= j0 + 1) {
             const Cons f = riemann<SOLVER, false>(hu, hw, lo.u, lo.w, ph);  // face (r-2 | r-1)
+            ur = src.load((r < j1 + 1 ? r + 1 : j1 + 1) * g.tnx + ic);
             if (r >= j0 + 2) {
                 const int j = r - 2;
                 const int k = j * g.tnx + ic;
@@ -233,7 +252,6 @@ MHD_DI double ysweep_column(const double* __restrict__ Ut, double* __restrict__ 
         }
         hu = hi.u;
         hw = hi.w;
-        w0 = w1;
         w1 = w2;
         slot = slot == 2 ? 0 : slot + 1;
     }
