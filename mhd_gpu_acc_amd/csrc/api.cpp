@@ -53,11 +53,13 @@ int check_desc(const mhd_run_desc* d) {
     return MHD_OK;
 }
 
-// Host read of the device CFL maximum (a non-negative double as raw bits).
+// Host read of the device CFL maximum (a non-negative double as raw bits). After a
+// step the closing ghost fill has already stored it into host_acc; otherwise copy it.
 int read_acc(mhd_run* r, double* out) {
-    MHD_HIP(hipMemcpyAsync(r->host_acc, r->acc, sizeof(unsigned long long), hipMemcpyDeviceToHost, r->stream()));
+    if (!r->host_acc_fresh)
+        MHD_HIP(hipMemcpyAsync(r->host_acc, r->acc, sizeof(unsigned long long), hipMemcpyDeviceToHost, r->stream()));
     MHD_HIP(hipStreamSynchronize(r->stream()));
-    unsigned long long bits = *r->host_acc;
+    unsigned long long bits = __atomic_load_n(r->host_acc, __ATOMIC_ACQUIRE);
     std::memcpy(out, &bits, sizeof bits);
     return MHD_OK;
 }
@@ -69,6 +71,7 @@ namespace mhd {
 // compute_dt on a grid: reduction on the device, ratchet and dt on the host.
 int dt_from_state(mhd_run* r, const double* U, bool reuse, double* dt) {
     if (!reuse) {
+        r->host_acc_fresh = false;
         MHD_HIP(hipMemsetAsync(r->acc, 0, sizeof(unsigned long long), r->stream()));
         MHD_HIP(mhd::launch_cfl(U, r->d.nx, r->d.ny, (long long)r->cells, r->d.gamma, r->ch, r->acc, r->stream()));
     }
@@ -105,6 +108,7 @@ int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, doubl
     a.cus = r->cus;
     a.frame = r->frame;
     a.acc = want_cfl ? r->acc : nullptr;
+    a.acc_host = want_cfl ? r->host_acc_dev : nullptr;
     a.stream = r->stream();
     a.ev = nullptr;
     if (r->profiling) {
@@ -117,7 +121,9 @@ int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, doubl
         a.ev = r->events.data() + base;
         r->prof_steps += 1;
     }
+    r->host_acc_fresh = false;
     MHD_HIP(mhd::launch_step(a));
+    r->host_acc_fresh = want_cfl && mhd::fused_cfl();
     return MHD_OK;
 }
 
@@ -179,8 +185,12 @@ int mhd_run_create(const mhd_run_desc* desc, mhd_run** out) {
     hipError_t e = hipMalloc(&r->frame, 4 * (size_t)(desc->nx + desc->ny) * sizeof(double));
     if (e != hipSuccess) return cleanup(e, "hipMalloc(frame)");
     if ((e = hipMalloc(&r->acc, sizeof(unsigned long long))) != hipSuccess) return cleanup(e, "hipMalloc(acc)");
-    if ((e = hipHostMalloc(&r->host_acc, sizeof(unsigned long long), hipHostMallocDefault)) != hipSuccess)
+    if ((e = hipHostMalloc(&r->host_acc, sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent)) !=
+        hipSuccess)
         return cleanup(e, "hipHostMalloc");
+    if ((e = hipHostGetDevicePointer((void**)&r->host_acc_dev, r->host_acc, 0)) != hipSuccess)
+        return cleanup(e, "hipHostGetDevicePointer");
+    *r->host_acc = 0;
     if ((e = hipStreamSynchronize(r->stream())) != hipSuccess) return cleanup(e, "hipStreamSynchronize");
     *out = r;
     return MHD_OK;

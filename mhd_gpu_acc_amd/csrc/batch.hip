@@ -139,10 +139,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
 }
 
 // phase 0: ghosts of U* (previous grid = the state); phase 1: ghosts of the next
-// state (previous grid = U*, psi from the undamped frame).
+// state (previous grid = U*, psi from the undamped frame), and block 0 hands the
+// runs' CFL maxima to the host: system-scope stores into pinned memory replace a
+// device-to-host copy after the step.
 __global__ __launch_bounds__(256) void batch_bc_kernel(const BatchRun* __restrict__ runs,
                                                        const BatchStep* __restrict__ steps,
-                                                       const int2* __restrict__ tiles, int phase) {
+                                                       const int2* __restrict__ tiles, int phase,
+                                                       const unsigned long long* __restrict__ acc,
+                                                       unsigned long long* acc_host, int nruns) {
+    if (acc_host && blockIdx.x == 0)
+        for (int r = threadIdx.x; r < nruns; r += blockDim.x)
+            __hip_atomic_store(acc_host + r, acc[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     const int2 tl = tiles[blockIdx.x];
     const BatchRun& R = runs[tl.x];
     const BatchStep& P = steps[tl.x];
@@ -189,7 +196,8 @@ struct mhd_batch {
     BatchStep* d_steps = nullptr;
     BatchStep* h_steps = nullptr;        // pinned
     unsigned long long* d_acc = nullptr;
-    unsigned long long* h_acc = nullptr; // pinned
+    unsigned long long* h_acc = nullptr; // pinned, coherent; written by the step's last ghost fill
+    unsigned long long* h_acc_dev = nullptr;  // its device address
     int cur = 0;
     bool acc_valid = false;
     // launch shape (autotuned)
@@ -263,12 +271,14 @@ template <int S, int L, int WG> void launch_bx(mhd_batch* b) {
 template <int S, int L> hipError_t launch_batch_step_sl(mhd_batch* b) {
     if (b->xwg == 128) launch_bx<S, L, 128>(b);
     else launch_bx<S, L, 256>(b);
-    hipLaunchKernelGGL(batch_bc_kernel, dim3(b->nbt), dim3(256), 0, b->stream, b->d_runs, b->d_steps, b->d_bt, 0);
+    hipLaunchKernelGGL(batch_bc_kernel, dim3(b->nbt), dim3(256), 0, b->stream, b->d_runs, b->d_steps, b->d_bt, 0,
+                       (const unsigned long long*)nullptr, (unsigned long long*)nullptr, 0);
     const int wpb = 4;
     const int nb = std::min((b->nyt + wpb - 1) / wpb, b->ygrid * b->cus);
     hipLaunchKernelGGL((batch_ysweep_kernel<S, L>), dim3(std::max(nb, 1)), dim3(64 * wpb), 0, b->stream, b->d_runs,
                        b->d_steps, b->d_yt, b->nyt, b->d_queue, b->d_acc);
-    hipLaunchKernelGGL(batch_bc_kernel, dim3(b->nbt), dim3(256), 0, b->stream, b->d_runs, b->d_steps, b->d_bt, 1);
+    hipLaunchKernelGGL(batch_bc_kernel, dim3(b->nbt), dim3(256), 0, b->stream, b->d_runs, b->d_steps, b->d_bt, 1,
+                       (const unsigned long long*)b->d_acc, b->h_acc_dev, b->n);
     return hipGetLastError();
 }
 
@@ -293,14 +303,13 @@ void drop_graph(mhd_batch* b) {
     b->graph = nullptr;
 }
 
-// One step as a graph: parameter upload, the four launches, CFL maxima read-back.
+// One step as a graph: parameter upload and the four launches; the last one stores
+// the CFL maxima into h_acc.
 int capture_graph(mhd_batch* b) {
     drop_graph(b);
     B_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
     hipError_t e = hipMemcpyAsync(b->d_steps, b->h_steps, b->n * sizeof(BatchStep), hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess) e = launch_batch_step(b);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b->h_acc, b->d_acc, b->n * sizeof(unsigned long long), hipMemcpyDeviceToHost, b->stream);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(b->stream, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -422,7 +431,9 @@ int mhd_batch_create(const mhd_run_desc* descs, int n, mhd_batch** out) {
         hipMalloc(&b->d_acc, n * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&b->d_queue, sizeof(unsigned int)) != hipSuccess ||
         hipHostMalloc(&b->h_steps, n * sizeof(BatchStep), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(&b->h_acc, n * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc(&b->h_acc, n * sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent) !=
+            hipSuccess ||
+        hipHostGetDevicePointer((void**)&b->h_acc_dev, b->h_acc, 0) != hipSuccess)
         return bail(set_error(MHD_ERR_HIP, "batch allocation"));
     std::memset(b->h_steps, 0, n * sizeof(BatchStep));
     if (int rc = build_tiles(b)) return bail(rc);

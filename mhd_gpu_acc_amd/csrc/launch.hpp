@@ -25,6 +25,8 @@ struct StepArgs {
     int cus;                 // compute units of the device
     double* frame;           // undamped psi of the interior frame (2*2*(nx+ny) doubles)
     unsigned long long* acc; // max signal speed accumulator (bit pattern of a double >= 0)
+    unsigned long long* acc_host;  // null, or the device address of pinned host memory that
+                                   // the last ghost fill copies acc into
     hipStream_t stream;
     hipEvent_t* ev;          // null, or 5 events recorded around the 4 launches
 };
