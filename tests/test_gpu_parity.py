@@ -215,6 +215,31 @@ def test_compute_dt_and_advance_api():
     assert ch == o["ch"]
 
 
+def test_cfl_maximum_handoff_after_steps():
+    """compute_dt after a step takes the CFL maximum that the step's closing ghost fill
+    stored into pinned host memory.  It must equal compute_dt of the same state and c_h
+    through the stand-alone reduction, also after two steps with no compute_dt between."""
+    cfg = m.get_n_case_config("orszag_tang_unit", 1)
+    cfg.nx, cfg.ny = 96, 80
+    U0 = m.initial_state(4, cfg)
+    say("[gpu] CFL hand-off")
+    with m.Run(cfg, m.RiemannSolver.HLLD, m.Limiter.VAN_LEER) as run, \
+            m.Run(cfg, m.RiemannSolver.HLLD, m.Limiter.VAN_LEER) as fresh:
+        run.set_state(U0)
+        dt = run.compute_dt()
+        for k in range(4):
+            run.advance(dt)
+            if k % 2:
+                run.advance(0.5 * dt)
+            ch = run.ch
+            dt = run.compute_dt()
+            fresh.set_state(run.get_state())
+            fresh.ch = ch
+            dt_fresh = fresh.compute_dt()
+            assert dt.hex() == dt_fresh.hex(), (k, dt, dt_fresh)
+            assert run.ch.hex() == fresh.ch.hex()
+
+
 def test_grid_level_api_with_torch_buffers():
     """mhd_advance_grids / mhd_compute_dt_grid over caller-owned device grids."""
     import torch
