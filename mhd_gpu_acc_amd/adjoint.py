@@ -52,6 +52,7 @@ class Tape:
         check(L.mhd_tape_create(run.handle, int(nsteps), int(K), C.byref(h)))
         self._h = h
         self.run = run
+        self.owner = None  # token of the autograd graph whose forward this tape recorded
         if not hasattr(run, "_tapes"):
             run._tapes = weakref.WeakSet()
         run._tapes.add(self)  # closed with the run
@@ -147,16 +148,21 @@ def _function():
             out = torch.empty_like(U0)
             run.get_state_device(out.data_ptr())
             ctx.tape = tape
+            ctx.token = tape.owner = object()  # this graph's claim on the tape's recording
             return out
 
         @staticmethod
         def backward(ctx, grad):
+            tape = ctx.tape
+            if tape._h is None or tape.owner is not ctx.token:
+                raise RuntimeError("MHDSteps: this graph's tape was closed or reused by a later forward "
+                                   "after its first backward; run the forward again to differentiate again")
             grad = grad.contiguous()
             torch.cuda.current_stream().synchronize()
             lam0 = torch.empty_like(grad)
-            tape = ctx.tape
             tape.backward_device(grad.data_ptr(), lam0.data_ptr())
-            ctx.tape = None
+            # The recording stays valid (a second backward, with retain_graph, recomputes
+            # from the same checkpoints) until a later forward takes the tape from the pool.
             tape.run.__dict__.setdefault("_idle_tapes", {})[(tape.nsteps, tape.K)] = tape
             return lam0, None, None, None
 

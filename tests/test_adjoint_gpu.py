@@ -185,6 +185,39 @@ def test_autograd_function_and_finite_differences():
     assert np.linalg.norm(adj - fd) <= 1e-6 * np.linalg.norm(fd), rel
 
 
+@pytest.mark.gpu
+def test_autograd_repeated_backward_and_tape_reuse():
+    """A second backward of one graph (retain_graph) gives the first gradient bit for bit;
+    once a later forward has taken that graph's tape from the idle pool, a further
+    backward of the old graph is refused with an error instead of using the new
+    recording.  Two graphs alive at once each keep their own tape."""
+    import torch
+    from mhd_gpu_acc_amd.adjoint import run_steps, to_planes
+    say("[gpu] autograd repeated backward, OT 32^2 HLLD van Leer 6 steps K=2")
+    case = small_case(32, 32, (0, 0, 0, 0))
+    U0 = perturbed_ot(case, 5)
+    with m.Run(cfg_of(case), 2, 1) as run:
+        run.ch = 0.0
+        a = to_planes(U0).requires_grad_(True)
+        la = (run_steps(a, run, 6, 2)[0] ** 2).sum()
+        g1, = torch.autograd.grad(la, a, retain_graph=True)
+        g2, = torch.autograd.grad(la, a, retain_graph=True)
+        assert torch.equal(g1, g2)
+        # two graphs alive at once: b takes the idle tape, c gets a tape of its own
+        run.ch = 0.0
+        b = to_planes(U0 * np.r_[1.001, 1, 1, 1, 1, 1, 1, 1, 1]).requires_grad_(True)
+        lb = (run_steps(b, run, 6, 2)[0] ** 2).sum()
+        run.ch = 0.0
+        c = to_planes(U0).requires_grad_(True)
+        lc = (run_steps(c, run, 6, 2)[0] ** 2).sum()
+        gc, = torch.autograd.grad(lc, c)
+        assert torch.equal(gc, g1)  # c recorded the same trajectory as a
+        gb, = torch.autograd.grad(lb, b)
+        assert not torch.equal(gb, g1)
+        with pytest.raises(RuntimeError, match="reused by a later forward"):
+            torch.autograd.grad(la, a)
+
+
 def _workload_gradient(run, U0, target, steps):
     from mhd_gpu_acc_amd.adjoint import from_planes, run_steps, to_planes
     run.ch = 0.0
