@@ -32,14 +32,14 @@ struct Bc4 {
 };
 
 // The adjoint planes of a grid seen from a sweep (rotated frame).
-template <bool XDIR> struct LamPlanes {
-    Planes<XDIR, const double> p;
+template <bool XDIR, bool BUF> struct LamPlanes {
+    std::conditional_t<BUF, BufPlanes<XDIR>, Planes<XDIR, const double>> p;
     int tnx;
     MHD_DI LamPlanes(const double* L, long long ps, int tnx_) : p(L, ps), tnx(tnx_) {}
     MHD_DI Cons operator()(int i, int j) const { return p.load(j * tnx + i); }
 };
 
-template <int SOLVER, int LIMITER, bool XDIR, int NL, int WG>
+template <int SOLVER, int LIMITER, bool XDIR, int NL, int WG, bool BUF>
 __global__ __launch_bounds__(WG) void adj_sweep_kernel(const double* __restrict__ U, const double* __restrict__ Lin,
                                                        double* __restrict__ Lout, Geom g, Phys ph, double dtd,
                                                        double hdtd, double psi_scale, Bc4 bc, int ntiles) {
@@ -88,8 +88,8 @@ __global__ __launch_bounds__(WG) void adj_sweep_kernel(const double* __restrict_
     }
     const AdjFlags fl = adj_flags(a, n, lineok);
     const int kc = j * g.tnx + i;
-    const Planes<XDIR, const double> src(U, g.ps);
-    const LamPlanes<XDIR> lin(Lin, g.ps, g.tnx);
+    const std::conditional_t<BUF, BufPlanes<XDIR>, Planes<XDIR, const double>> src(U, g.ps);
+    const LamPlanes<XDIR, BUF> lin(Lin, g.ps, g.tnx);
     const int tm = pos > 0 ? t - NL : t;
     const int tp = pos < LT - 1 ? t + NL : t;
 
@@ -138,7 +138,8 @@ __global__ __launch_bounds__(WG) void adj_sweep_kernel(const double* __restrict_
     const Cons mc = pos < LT - 1 ? sA.get(tp) : cons_zero();
     const Prim mnext{mc.r, mc.mn, mc.mt, mc.mz, mc.bn, mc.bt, mc.bz, mc.e, mc.psi, 0.0};
     const Cons out = adj_cell(u, w, pprev, wcb, mnext, fl.interior, gown, ph);
-    if (inb && lineok && pos >= 3 && pos < LT - 3) Planes<XDIR, double>(Lout, g.ps).store(kc, out);
+    if (inb && lineok && pos >= 3 && pos < LT - 3)
+        std::conditional_t<BUF, BufPlanes<XDIR>, Planes<XDIR, double>>(Lout, g.ps).store(kc, out);
 }
 
 // Tile shapes (compile-time knobs): workgroup width and, for the y pass, columns
@@ -155,8 +156,8 @@ __global__ __launch_bounds__(WG) void adj_sweep_kernel(const double* __restrict_
 #define MHD_ADJ_YNL 4
 #endif
 
-template <int S, int L>
-static hipError_t launch_adjoint_sl(const AdjArgs& a) {
+template <int S, int L, bool BUF>
+static hipError_t launch_adjoint_sla(const AdjArgs& a) {
     const Geom g{a.nx, a.ny, a.nx + 2 * NG, a.ny + 2 * NG, a.ps};
     const Phys ph{a.gamma, a.gamma - 1.0, a.ch, a.ch * a.ch, 0.5 * a.ch, 0.5 / a.ch};
     const double dtdx = a.dt / a.dx, dtdy = a.dt / a.dy;
@@ -165,16 +166,23 @@ static hipError_t launch_adjoint_sl(const AdjArgs& a) {
     constexpr int XWG = MHD_ADJ_XWG, YWG = MHD_ADJ_YWG, NLY = MHD_ADJ_YNL;
     constexpr int NOUTY = YWG / NLY - 6, NOUTX = XWG - 6;
     dim3 gy((unsigned)((g.nx + NLY - 1) / NLY), (unsigned)((g.tny + NOUTY - 1) / NOUTY));
-    hipLaunchKernelGGL((adj_sweep_kernel<S, L, false, NLY, YWG>), gy, dim3(YWG), 0, a.stream, a.Utmp, a.lam,
+    hipLaunchKernelGGL((adj_sweep_kernel<S, L, false, NLY, YWG, BUF>), gy, dim3(YWG), 0, a.stream, a.Utmp, a.lam,
                        a.lam_tmp, g, ph, dtdy, 0.5 * dtdy, damping ? a.damp : 1.0, bc, 0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const long long flat = (long long)g.tnx * g.ny;
     const int ntiles = (int)((flat + NOUTX - 1) / NOUTX);
     const int nzero = (2 * NG * g.tnx + XWG - 1) / XWG;
-    hipLaunchKernelGGL((adj_sweep_kernel<S, L, true, 1, XWG>), dim3((unsigned)(ntiles + nzero)), dim3(XWG), 0,
+    hipLaunchKernelGGL((adj_sweep_kernel<S, L, true, 1, XWG, BUF>), dim3((unsigned)(ntiles + nzero)), dim3(XWG), 0,
                        a.stream, a.Uold, a.lam_tmp, a.lam, g, ph, dtdx, 0.5 * dtdx, 1.0, bc, ntiles);
     return hipGetLastError();
+}
+
+// Buffer-resource addressing for grids of at most 4 GiB, as the forward sweeps.
+template <int S, int L>
+static hipError_t launch_adjoint_sl(const AdjArgs& a) {
+    if (a.addr_buf && 9ull * (unsigned long long)a.ps * 8 <= BUF_MAX_BYTES) return launch_adjoint_sla<S, L, true>(a);
+    return launch_adjoint_sla<S, L, false>(a);
 }
 
 hipError_t launch_adjoint(const AdjArgs& a) {

@@ -26,15 +26,15 @@
 
 namespace mhd {
 
-template <int SOLVER, int LIMITER, int WG>
+template <int SOLVER, int LIMITER, int WG, bool BUF>
 __global__ __launch_bounds__(WG) void xsweep_kernel(const double* __restrict__ Uo, double* __restrict__ Ut, Geom g,
                                                     Phys ph, double dtd, double hdtd,
                                                     unsigned long long* __restrict__ acc, int nseg) {
     __shared__ XLds<WG> L;
     const int t = threadIdx.x;
     if (acc && blockIdx.x == 0 && t == 0) *acc = 0ull;  // CFL accumulator for this step's ysweep
-    const Planes<true, const double> src(Uo, g.ps);
-    const Planes<true, double> dst(Ut, g.ps);
+    const std::conditional_t<BUF, BufPlanes<true>, Planes<true, const double>> src(Uo, g.ps);
+    const std::conditional_t<BUF, BufPlanes<true>, Planes<true, double>> dst(Ut, g.ps);
     const int last = g.tnx * g.tny - 1;
     const int kb = NG * g.tnx - 2 + t;
     int s = blockIdx.x;
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(WG) void xsweep_kernel(const double* __restrict__ U
     }
 }
 
-template <int SOLVER, int LIMITER>
+template <int SOLVER, int LIMITER, bool BUF>
 __global__ __launch_bounds__(256) MHD_Y_ATTR void ysweep_kernel(const double* __restrict__ Ut,
                                                                 double* __restrict__ Un, Geom g, Phys ph,
                                                                 double dtd, double hdtd, double damp, int seg,
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) MHD_Y_ATTR void ysweep_kernel(const double* __
     const int i = NG + blockIdx.x * blockDim.x + threadIdx.x;
     const int j0 = NG + blockIdx.y * seg;
     const int j1 = min(j0 + seg, NG + g.ny);
-    const double cand = ysweep_column<SOLVER, LIMITER>(Ut, Un, g, ph, dtd, hdtd, damp, frame, acc != nullptr, i,
+    const double cand = ysweep_column<SOLVER, LIMITER, BUF>(Ut, Un, g, ph, dtd, hdtd, damp, frame, acc != nullptr, i,
                                                        i < NG + g.nx, j0, j1,
                                                        YRing{ring, (int)threadIdx.x, (int)blockDim.x});
     if (acc) wave_max_to(acc, cand);
@@ -127,14 +127,14 @@ namespace mhd {
 
 bool fused_cfl() { return MHD_FUSED_CFL != 0; }
 
-template <int S, int L, int WG>
+template <int S, int L, int WG, bool BUF>
 static void launch_x(const StepArgs& a, const Geom& g, const Phys& ph, double dtdx) {
     const long long flat = (long long)g.tnx * g.ny;
     const int nseg = (int)((flat + (WG - 4) - 1) / (WG - 4));
     // xgrid > 0: persistent grid of xgrid blocks per CU; 0: one block per segment
     const int nb = a.xgrid > 0 ? min(nseg, a.xgrid * a.cus) : nseg;
-    hipLaunchKernelGGL((xsweep_kernel<S, L, WG>), dim3((unsigned)nb), dim3(WG), 0, a.stream, a.Uold, a.Utmp, g, ph,
-                       dtdx, 0.5 * dtdx, a.acc, nseg);
+    hipLaunchKernelGGL((xsweep_kernel<S, L, WG, BUF>), dim3((unsigned)nb), dim3(WG), 0, a.stream, a.Uold, a.Utmp, g,
+                       ph, dtdx, 0.5 * dtdx, a.acc, nseg);
 }
 
 template <int S, int L>
@@ -143,10 +143,12 @@ static hipError_t launch_step_sl(const StepArgs& a) {
     Phys ph{a.gamma, a.gamma - 1.0, a.ch, a.ch * a.ch, 0.5 * a.ch, 0.5 / a.ch};
     const double dtdx = a.dt / a.dx, dtdy = a.dt / a.dy;
     hipError_t e;
+    // Buffer-resource addressing for grids of at most 4 GiB (an OT 4096^2 grid is 1.2 GB).
+    const bool buf = a.addr_buf && 9ull * (unsigned long long)a.ps * 8 <= BUF_MAX_BYTES;
     if (a.ev && (e = hipEventRecord(a.ev[0], a.stream)) != hipSuccess) return e;
-    if (a.xwg == 64) launch_x<S, L, 64>(a, g, ph, dtdx);
-    else if (a.xwg == 128) launch_x<S, L, 128>(a, g, ph, dtdx);
-    else launch_x<S, L, 256>(a, g, ph, dtdx);
+    if (a.xwg == 64) buf ? launch_x<S, L, 64, true>(a, g, ph, dtdx) : launch_x<S, L, 64, false>(a, g, ph, dtdx);
+    else if (a.xwg == 128) buf ? launch_x<S, L, 128, true>(a, g, ph, dtdx) : launch_x<S, L, 128, false>(a, g, ph, dtdx);
+    else buf ? launch_x<S, L, 256, true>(a, g, ph, dtdx) : launch_x<S, L, 256, false>(a, g, ph, dtdx);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.ev && (e = hipEventRecord(a.ev[1], a.stream)) != hipSuccess) return e;
     const int nbc = 2 * NG * g.ny + 2 * NG * g.tnx;
@@ -165,8 +167,12 @@ static hipError_t launch_step_sl(const StepArgs& a) {
     }
     const int ywg = (a.ywg == 64 || a.ywg == 128) ? a.ywg : 256;
     dim3 gy((unsigned)((g.nx + ywg - 1) / ywg), (unsigned)((g.ny + seg - 1) / seg));
-    hipLaunchKernelGGL((ysweep_kernel<S, L>), gy, dim3(ywg), 0, a.stream, a.Utmp, a.Unew, g, ph, dtdy, 0.5 * dtdy,
-                       a.damp, seg, a.frame, MHD_FUSED_CFL ? a.acc : nullptr);
+    if (buf)
+        hipLaunchKernelGGL((ysweep_kernel<S, L, true>), gy, dim3(ywg), 0, a.stream, a.Utmp, a.Unew, g, ph, dtdy,
+                           0.5 * dtdy, a.damp, seg, a.frame, MHD_FUSED_CFL ? a.acc : nullptr);
+    else
+        hipLaunchKernelGGL((ysweep_kernel<S, L, false>), gy, dim3(ywg), 0, a.stream, a.Utmp, a.Unew, g, ph, dtdy,
+                           0.5 * dtdy, a.damp, seg, a.frame, MHD_FUSED_CFL ? a.acc : nullptr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.ev && (e = hipEventRecord(a.ev[3], a.stream)) != hipSuccess) return e;
     const bool to_host = MHD_FUSED_CFL && a.acc && a.acc_host;

@@ -22,6 +22,7 @@ struct StepArgs {
     int xwg;                 // x-sweep workgroup width: 64, 128 or 256 (0 = default)
     int ywg;                 // y-sweep workgroup width: 64, 128 or 256 (0 = default)
     int xgrid;               // x-sweep blocks per CU (persistent), 0 = one block per segment
+    bool addr_buf;           // sweeps address through buffer resources where the grid allows (else flat)
     int cus;                 // compute units of the device
     double* frame;           // undamped psi of the interior frame (2*2*(nx+ny) doubles)
     unsigned long long* acc; // max signal speed accumulator (bit pattern of a double >= 0)
@@ -47,6 +48,7 @@ struct AdjArgs {
     double damp;             // the step's psi damping factor
     int solver, limiter;     // solver 0 (HLL) or 2 (HLLD)
     int bc[4];               // no Dirichlet side
+    bool addr_buf;           // address through buffer resources where the grid allows (else flat)
     hipStream_t stream;
 };
 hipError_t launch_adjoint(const AdjArgs& a);

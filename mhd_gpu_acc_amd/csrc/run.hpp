@@ -26,6 +26,7 @@ struct mhd_run {
     int xwg = 0;
     int ywg = 0;
     int xgrid = 0;
+    bool addr_buf = true;
     int cus = 1;
     bool profiling = false;
     std::vector<hipEvent_t> events;  // 5 per profiled step

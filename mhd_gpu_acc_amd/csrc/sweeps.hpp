@@ -6,6 +6,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "physics.hpp"
 
 #pragma clang fp contract(off)
@@ -48,6 +50,43 @@ MHD_DI Cons load_rot(const double* __restrict__ U, long long ps, long long k) {
 // traffic.
 template <class T> using GlobalT = __attribute__((address_space(1))) T;
 template <class T> MHD_DI GlobalT<T>* as_global(T* p) { return (GlobalT<T>*)p; }
+
+// Buffer-resource view of the nine planes of a grid of at most 4 GiB - 1 byte: one
+// resource (4 SGPRs) for the grid and the plane offsets as scalar offsets, so the nine
+// accesses of a cell share one 32-bit lane offset instead of each forming a 64-bit
+// address.  Every access is in range by construction.
+constexpr unsigned long long BUF_MAX_BYTES = 0xFFFFFFFFull;
+template <bool XDIR> struct BufPlanes {
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    __amdgpu_buffer_rsrc_t rs;
+    unsigned ps8;  // plane stride in bytes
+    MHD_DI BufPlanes(const double* U, long long ps)
+        : rs(__builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(U), 0, (int)(unsigned)(9ull * ps * 8), 0x00020000)),
+          ps8((unsigned)(ps * 8)) {}
+    template <int F> MHD_DI double ld(unsigned vo) const {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)vo, (int)(F * ps8), 0));
+    }
+    template <int F> MHD_DI void st(unsigned vo, double v) const {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), rs, (int)vo, (int)(F * ps8), 0);
+    }
+    MHD_DI Cons load(int k) const {
+        const unsigned vo = (unsigned)k * 8u;
+        return Cons{ld<F_RHO>(vo), ld<XDIR ? F_MX : F_MY>(vo), ld<XDIR ? F_MY : F_MX>(vo), ld<F_MZ>(vo),
+                    ld<XDIR ? F_BX : F_BY>(vo), ld<XDIR ? F_BY : F_BX>(vo), ld<F_BZ>(vo), ld<F_E>(vo),
+                    ld<F_PSI>(vo)};
+    }
+    MHD_DI void store(int k, const Cons& u) const {
+        const unsigned vo = (unsigned)k * 8u;
+        st<F_RHO>(vo, u.r); st<XDIR ? F_MX : F_MY>(vo, u.mn); st<XDIR ? F_MY : F_MX>(vo, u.mt); st<F_MZ>(vo, u.mz);
+        st<XDIR ? F_BX : F_BY>(vo, u.bn); st<XDIR ? F_BY : F_BX>(vo, u.bt); st<F_BZ>(vo, u.bz); st<F_E>(vo, u.e);
+        st<F_PSI>(vo, u.psi);
+    }
+};
+
+template <bool BUF, class V, class T> MHD_DI V plane_view(T* U, long long ps) {
+    if constexpr (BUF) return V(U, ps);
+    else return V(as_global(U), ps);
+}
 
 // Workgroup barrier that orders LDS only.  __syncthreads() fences global memory
 // too, which makes every lane wait for its in-flight global loads (the next
@@ -204,12 +243,15 @@ struct YRing {
 // (0 when want_cfl is false or the lane is idle).
 // Ut / Un are the bases of the source and destination grids: the restrict
 // qualifiers let the next rows' loads run ahead of this row's stores.
-template <int SOLVER, int LIMITER>
+// BUF: address both grids through buffer resources (each grid at most BUF_MAX_BYTES).
+template <int SOLVER, int LIMITER, bool BUF = false>
 MHD_DI double ysweep_column(const double* __restrict__ Ut, double* __restrict__ Un, const Geom& g, const Phys& ph,
                             double dtd, double hdtd, double damp, double* __restrict__ frame, bool want_cfl, int i,
                             bool active, int j0, int j1, const YRing& ring) {
-    const Planes<false, GlobalT<const double>> src(as_global(Ut), g.ps);
-    const Planes<false, GlobalT<double>> dst(as_global(Un), g.ps);
+    using Src = std::conditional_t<BUF, BufPlanes<false>, Planes<false, GlobalT<const double>>>;
+    using Dst = std::conditional_t<BUF, BufPlanes<false>, Planes<false, GlobalT<double>>>;
+    const Src src = plane_view<BUF, Src>(Ut, g.ps);
+    const Dst dst = plane_view<BUF, Dst>(Un, g.ps);
     const int ic = active ? i : NG;  // idle lanes shadow column NG (no stores)
     const bool damping = !(damp >= 1.0);  // solver_cpu.cpp:265 skips only when factor >= 1
     double cand = 0.0;

@@ -127,6 +127,7 @@ int reverse_step(mhd_run* r, const double* Uold, const double* Utmp, double dt, 
     a.solver = r->d.solver;
     a.limiter = r->d.limiter;
     std::memcpy(a.bc, r->d.bc, sizeof a.bc);
+    a.addr_buf = r->addr_buf;
     a.stream = r->stream();
     MHD_HIP(mhd::launch_adjoint(a));
     return MHD_OK;

@@ -69,6 +69,14 @@ def test_reverse_step_matches_host(nx, ny, bc, solver, limiter):
     assert_bitwise(got, expected, "reverse step")
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("nx,ny,bc,solver,limiter", [(70, 70, (0, 0, 0, 0), 2, 1), (19, 66, (2, 1, 0, 0), 1, 0)])
+def test_reverse_step_flat_addressing_matches_host(nx, ny, bc, solver, limiter, monkeypatch):
+    """The flat-addressed reverse kernels (the path of grids above 4 GiB), forced."""
+    monkeypatch.setenv("MHD_ADDR", "flat")
+    test_reverse_step_matches_host(nx, ny, bc, solver, limiter)
+
+
 def _oracle_chain(case, U0, nsteps, solver, limiter):
     states, tmps, dts, chs = [U0], [], [], []
     ch, U = 0.0, U0
