@@ -115,6 +115,12 @@ int mhd_init_state_device(int case_id, const mhd_case_config* geom, const mhd_ic
  *      solver_gpu.cuh:9-35): allocates Uold/Utmp/Unew on the device. ------------ */
 int mhd_run_create(const mhd_run_desc* desc, mhd_run** out);
 int mhd_run_destroy(mhd_run* run);
+/* The stream the run's work is queued on from the next call on (the descriptor's
+ * `stream`, NULL = the null stream).  The reference queues everything on the legacy
+ * default stream (solver_gpu.cu:595,602,682); here a caller such as a framework's
+ * current stream can take over a run: the new stream first waits, on the device, for
+ * the work already queued on the old one. */
+int mhd_run_set_stream(mhd_run* run, void* stream);
 size_t mhd_plane_stride(const mhd_run* run);   /* doubles between planes */
 /* Device base pointer of the current state (9 planes, plane_stride apart). */
 double* mhd_state_ptr(mhd_run* run);

@@ -11,7 +11,6 @@
 #ifndef MHD_HIP_HPP
 #define MHD_HIP_HPP
 
-#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -48,7 +47,7 @@ public:
     Run(int nx, int ny, double x_min, double x_max, double y_min, double y_max, double gamma, double cfl,
         RiemannSolver solver, const BoundaryConfig& bc, Limiter limiter = Limiter::Minmod, double cr = 0.18,
         void* stream = nullptr)
-        : dx_((x_max - x_min) / static_cast<double>(nx)), dy_((y_max - y_min) / static_cast<double>(ny)) {
+        : cfl_(cfl) {
         mhd_run_desc d{};
         d.nx = nx;
         d.ny = ny;
@@ -72,11 +71,10 @@ public:
     ~Run() { mhd_run_destroy(run_); }
     Run(const Run&) = delete;
     Run& operator=(const Run&) = delete;
-    Run(Run&& o) noexcept : run_(std::exchange(o.run_, nullptr)), dx_(o.dx_), dy_(o.dy_) {}
+    Run(Run&& o) noexcept : run_(std::exchange(o.run_, nullptr)), cfl_(o.cfl_) {}
     Run& operator=(Run&& o) noexcept {
         std::swap(run_, o.run_);
-        std::swap(dx_, o.dx_);
-        std::swap(dy_, o.dy_);
+        std::swap(cfl_, o.cfl_);
         return *this;
     }
 
@@ -89,12 +87,21 @@ public:
     void download_device_soa(double* U) { check(mhd_get_state(run_, U, 2)); }
 
     /* compute_dt (solver_cpu.hpp:49) / compute_dt_gpu (solver_gpu.cuh:40): ratchets
-     * c_h to the maximum signal speed and returns (cfl * min(dx, dy)) / c_h. */
+     * c_h to the maximum signal speed and returns the library's (cfl * min(dx, dy)) / c_h.
+     * The run's CFL number is fixed when it is constructed; a different `cfl` throws
+     * std::invalid_argument rather than silently computing a second dt. */
     double compute_dt(double cfl) {
+        if (!(cfl == cfl_))
+            throw std::invalid_argument("mhd_hip: compute_dt(" + std::to_string(cfl) + ") on a run built with CFL " +
+                                        std::to_string(cfl_));
+        return compute_dt();
+    }
+    double compute_dt() {
         double dt = 0.0;
         check(mhd_compute_dt(run_, &dt));
-        return cfl * std::min(dx_, dy_) / ch();
+        return dt;
     }
+    double cfl() const { return cfl_; }
     /* advance_second_order(_gpu) (solver_cpu.hpp:55-63, solver_gpu.cuh:45-53) with the
      * run's solver and BCs, followed by the caller's swap (main_gpu.cu:236). */
     void advance_second_order(double dt) { check(mhd_advance(run_, dt)); }
@@ -113,6 +120,8 @@ public:
     }
     void set_ch(double c) { check(mhd_set_ch(run_, c)); }
     void synchronize() { check(mhd_sync(run_)); }
+    /* Queue the run's work on `stream` (a hipStream_t) from now on (mhd_run_set_stream). */
+    void set_stream(void* stream) { check(mhd_run_set_stream(run_, stream)); }
 
     /* initialise_grid + upload_from_aos (init.cpp:34-43) computed on the device, for the
      * cases that have a device state (mhd_run_init_state). */
@@ -164,7 +173,7 @@ public:
 
 private:
     mhd_run* run_ = nullptr;
-    double dx_, dy_;  // grid_cpu.hpp:25-26
+    double cfl_;
 };
 
 }  // namespace mhd_hip

@@ -70,6 +70,7 @@ def lib():
     L.mhd_state_ptr.argtypes = [C.c_void_p]
     L.mhd_run_create.argtypes = [C.POINTER(RunDesc), C.POINTER(C.c_void_p)]
     L.mhd_run_destroy.argtypes = [C.c_void_p]
+    L.mhd_run_set_stream.argtypes = [C.c_void_p, C.c_void_p]
     L.mhd_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
     L.mhd_get_state.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
     L.mhd_set_ch.argtypes = [C.c_void_p, C.c_double]

@@ -127,6 +127,22 @@ def _check_tensor(U, run: Run):
         raise ValueError(f"state shape {tuple(U.shape)} != {shape}")
 
 
+class _on_current_stream:
+    """Queue the run's work on torch's current stream for the duration (stream order,
+    no host synchronisation of torch's streams); the run's own stream afterwards."""
+
+    def __init__(self, run: Run):
+        self.run = run
+
+    def __enter__(self):
+        import torch
+        self.prev = self.run.stream
+        self.run.set_stream(torch.cuda.current_stream().cuda_stream)
+
+    def __exit__(self, *exc):
+        self.run.set_stream(self.prev)
+
+
 def _function():
     import torch
 
@@ -137,16 +153,17 @@ def _function():
         def forward(ctx, U0, run: Run, nsteps: int, K: int = 20):
             _check_tensor(U0, run)
             U0 = U0.contiguous()
-            torch.cuda.current_stream().synchronize()
-            run.set_state_device(U0.data_ptr())
             # an idle tape of the same shape (one whose backward has run) is reused
             pool = run.__dict__.setdefault("_idle_tapes", {})
             tape = pool.pop((nsteps, K), None)
             if tape is None or tape._h is None:
                 tape = Tape(run, nsteps, K)
-            tape.forward()
+            tape.owner = None  # a recording interrupted below belongs to no graph
             out = torch.empty_like(U0)
-            run.get_state_device(out.data_ptr())
+            with _on_current_stream(run):
+                run.set_state_device(U0.data_ptr())
+                tape.forward()
+                run.get_state_device(out.data_ptr())
             ctx.tape = tape
             ctx.token = tape.owner = object()  # this graph's claim on the tape's recording
             return out
@@ -158,12 +175,14 @@ def _function():
                 raise RuntimeError("MHDSteps: this graph's tape was closed or reused by a later forward "
                                    "after its first backward; run the forward again to differentiate again")
             grad = grad.contiguous()
-            torch.cuda.current_stream().synchronize()
             lam0 = torch.empty_like(grad)
-            tape.backward_device(grad.data_ptr(), lam0.data_ptr())
+            with _on_current_stream(tape.run):
+                tape.backward_device(grad.data_ptr(), lam0.data_ptr())
             # The recording stays valid (a second backward, with retain_graph, recomputes
-            # from the same checkpoints) until a later forward takes the tape from the pool.
-            tape.run.__dict__.setdefault("_idle_tapes", {})[(tape.nsteps, tape.K)] = tape
+            # from the same checkpoints) until a later forward takes the tape from the
+            # pool.  The pool keeps one idle tape per run: a tape it displaces is closed
+            # once no graph refers to it any more (Tape.__del__).
+            tape.run.__dict__["_idle_tapes"] = {(tape.nsteps, tape.K): tape}
             return lam0, None, None, None
 
     return MHDSteps
@@ -193,13 +212,14 @@ def replay(run: Run, U0, dts, chs):
     with frozen dt / c_h use it)."""
     import torch
     _check_tensor(U0, run)
-    torch.cuda.current_stream().synchronize()
-    run.set_state_device(U0.contiguous().data_ptr())
-    for dt, ch in zip(dts, chs):
-        run.ch = float(ch)
-        run.advance(float(dt))
+    U0 = U0.contiguous()
     out = torch.empty_like(U0)
-    run.get_state_device(out.data_ptr())
+    with _on_current_stream(run):
+        run.set_state_device(U0.data_ptr())
+        for dt, ch in zip(dts, chs):
+            run.ch = float(ch)
+            run.advance(float(dt))
+        run.get_state_device(out.data_ptr())
     return out
 
 

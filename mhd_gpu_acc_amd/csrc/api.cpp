@@ -194,6 +194,8 @@ int mhd_run_create(const mhd_run_desc* desc, mhd_run** out) {
     if ((e = hipHostGetDevicePointer((void**)&r->host_acc_dev, r->host_acc, 0)) != hipSuccess)
         return cleanup(e, "hipHostGetDevicePointer");
     *r->host_acc = 0;
+    if ((e = hipEventCreateWithFlags(&r->order, hipEventDisableTiming)) != hipSuccess)
+        return cleanup(e, "hipEventCreate");
     if ((e = hipStreamSynchronize(r->stream())) != hipSuccess) return cleanup(e, "hipStreamSynchronize");
     *out = r;
     return MHD_OK;
@@ -203,12 +205,23 @@ int mhd_run_destroy(mhd_run* r) {
     if (!r) return MHD_OK;
     (void)hipStreamSynchronize(r->stream());
     for (hipEvent_t ev : r->events) (void)hipEventDestroy(ev);
+    if (r->order) (void)hipEventDestroy(r->order);
     for (double* p : r->U)
         if (p) (void)hipFree(p);
     if (r->frame) (void)hipFree(r->frame);
     if (r->acc) (void)hipFree(r->acc);
     if (r->host_acc) (void)hipHostFree(r->host_acc);
     delete r;
+    return MHD_OK;
+}
+
+int mhd_run_set_stream(mhd_run* r, void* stream) {
+    if (!r) return fail(MHD_ERR_INVALID, "null run");
+    if (stream == r->d.stream) return MHD_OK;
+    // The new stream waits for what is already queued on the old one (no host wait).
+    MHD_HIP(hipEventRecord(r->order, r->stream()));
+    MHD_HIP(hipStreamWaitEvent(static_cast<hipStream_t>(stream), r->order, 0));
+    r->d.stream = stream;
     return MHD_OK;
 }
 

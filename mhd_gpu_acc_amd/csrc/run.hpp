@@ -30,6 +30,7 @@ struct mhd_run {
     int cus = 1;
     bool profiling = false;
     std::vector<hipEvent_t> events;  // 5 per profiled step
+    hipEvent_t order = nullptr;      // orders a stream change (mhd_run_set_stream)
     int prof_steps = 0;
     hipStream_t stream() const { return static_cast<hipStream_t>(d.stream); }
 };

@@ -31,7 +31,6 @@ struct mhd_tape {
     mhd_run* run = nullptr;
     int nsteps = 0, K = 0, nck = 0;
     std::vector<double> dts, chs;
-    double ch_after = 0.0;
     bool recorded = false;
     bool last_kept = false;    // S/T hold the last segment as the forward stepped it
     std::vector<double*> ck;   // checkpoints
@@ -221,7 +220,6 @@ int mhd_tape_forward(mhd_tape* t, double* dts, double* chs) {
     MHD_HIP(hipMemcpyAsync(r->U[0], t->S[m], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
     MHD_HIP(hipStreamSynchronize(r->stream()));
     r->acc_valid = mhd::fused_cfl();  // acc holds the CFL maximum of S[m], now U[0]
-    t->ch_after = r->ch;
     t->recorded = true;
     t->last_kept = true;
     return MHD_OK;
@@ -231,6 +229,11 @@ int mhd_tape_backward(mhd_tape* t, const double* lam_final, double* lam0, int la
     if (!t || !lam_final || !lam0) return fail(MHD_ERR_INVALID, "null argument");
     if (!t->recorded) return fail(MHD_ERR_INVALID, "tape: no recorded forward");
     mhd_run* r = t->run;
+    // The recompute below sets the run's c_h step by step; the run itself (its state,
+    // c_h ratchet and CFL accumulator) is left as the caller had it, which need not be
+    // this tape's end (another tape's forward may have advanced the run since).
+    const double ch_run = r->ch;
+    const bool acc_valid = r->acc_valid, host_acc_fresh = r->host_acc_fresh;
     unsigned int bad = 0;
     if (int rc = nonfinite(t, t->ck[0], &bad)) return rc;
     if (bad) return fail(MHD_ERR_INVALID, "adjoint: the initial state is not finite");
@@ -255,8 +258,11 @@ int mhd_tape_backward(mhd_tape* t, const double* lam_final, double* lam0, int la
         for (int q = m - 1; q >= 0 && !rc; --q)
             rc = reverse_step(r, t->S[q], t->T[q], t->dts[s0 + q], t->chs[s0 + q], t->lam[0], t->lam[1]);
     }
-    r->ch = t->ch_after;
-    r->acc_valid = false;
+    // The recompute writes only the segment buffers and skips the CFL reduction, so the
+    // run's U[0], accumulator and pinned copy are untouched: restore the flags as well.
+    r->ch = ch_run;
+    r->acc_valid = acc_valid;
+    r->host_acc_fresh = host_acc_fresh;
     if (rc) return rc;
     return download(r, t->lam[0], layout, lam0);
 }

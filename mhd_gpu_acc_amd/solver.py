@@ -155,6 +155,7 @@ class Run:
         h = C.c_void_p()
         check(lib().mhd_run_create(C.byref(d), C.byref(h)))
         self._h = h
+        self._stream = int(stream or 0)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -178,6 +179,18 @@ class Run:
     @property
     def handle(self):
         return self._h
+
+    @property
+    def stream(self) -> int:
+        """hipStream_t (as an int, 0 = the null stream) the run's work is queued on."""
+        return self._stream
+
+    def set_stream(self, stream: int):
+        """Queue the run's work on ``stream`` from now on (mhd_run_set_stream): the new
+        stream first waits, on the device, for what is already queued on the old one."""
+        stream = int(stream or 0)
+        check(lib().mhd_run_set_stream(self._h, C.c_void_p(stream) if stream else None))
+        self._stream = stream
 
     @property
     def plane_stride(self) -> int:

@@ -10,7 +10,7 @@ runs are stepped by the oracle, which is pinned bitwise to the reference elsewhe
 the dt and c_h sequences (little-endian float64), the first and last dt as hex, SHA-256
 of the final state (host AoS, ghosts included, float64) and its per-field L1 norms.
 
-    make -C oracle && python tests/golden/make_fullsize_golden.py      (about an hour on 8 cores)
+    make -C oracle && python tests/golden/make_fullsize_golden.py      (about 80 minutes on 8 cores; the 7800^2 run needs about 40 GB of host memory)
 """
 from __future__ import annotations
 
@@ -43,6 +43,13 @@ RUNS = {
 for s in SOLVERS:
     for lim, lname in ((1, "vanleer"), (0, "minmod")):
         RUNS[f"ot1024_{s}_{lname}_t0.5"] = (OT_UNIT, 1024, 1024, s, lim, 100000, 0.5)
+# The headline size with the other three Riemann solvers, both limiters.
+for s in ("hll", "hllc", "force"):
+    for lim, lname in ((1, "vanleer"), (0, "minmod")):
+        RUNS[f"ot4096_{s}_{lname}_100"] = (OT_UNIT, 4096, 4096, s, lim, 100, None)
+# A grid whose nine planes exceed 4 GiB (7804^2 cells with ghosts, 4.38 GB): the sweeps
+# address it with 64-bit flat addresses instead of a buffer resource.
+RUNS["ot7800_hll_minmod_3"] = (OT_UNIT, 7800, 7800, "hll", 0, 3, None)
 # At CFL 0.8 the HLLC and HLLD runs stop at the blow-up guard before t = 0.5 (so does the
 # reference's own solver); at the reference's own Orszag-Tang CFL of 0.4 HLLD finishes.
 for lim, lname in ((1, "vanleer"), (0, "minmod")):

@@ -104,41 +104,98 @@ def test_no_device_is_an_error_not_a_fallback():
     assert e.value.code == 6
 
 
-def test_cpp_front_compiles_links_and_reports(tmp_path):
-    """include/mhd_hip.hpp (the head/-shaped C++ front) compiles with g++, links against
-    the library, and runs: without a GPU the run constructor throws the library's
-    no-device error (no CPU fallback); with one it steps Brio-Wu from its device initial
-    state and drives it to t_end in the reference driver's 978 steps."""
-    import subprocess
-    src = tmp_path / "front.cpp"
-    src.write_text(r'''
+_FRONT_SRC = r'''
 #include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
 #include "mhd_hip.hpp"
-int main() {
+// argv[1]: directory for the driver's CSV files
+int main(int argc, char** argv) {
     try {
         mhd_hip::BoundaryConfig bc;  // Brio-Wu: transmissive in x, periodic in y (test_cases.cpp)
         bc.bottom = bc.top = mhd_hip::BoundaryType::Periodic;
+        const double t_end = 0.1;
+        // the reference driver's loop (main_cpu.cpp:196-222) written against the front
         mhd_hip::Run run(800, 4, 0.0, 1.0, 0.0, 4.0 / 800.0, 2.0, 0.4, mhd_hip::RiemannSolver::HLLD, bc);
         run.init_state_device(MHD_CASE_BRIO_WU);
         double t = 0.0;
-        for (int s = 0; s < 5; ++s) { const double dt = run.compute_dt(0.4); run.advance_second_order(dt); t += dt; }
+        long long steps = 0;
+        while (t < t_end) {
+            double dt = run.compute_dt(0.4);
+            if (t_end - t < dt) dt = t_end - t;
+            run.advance_second_order(dt);
+            t += dt;
+            ++steps;
+        }
+        bool rejected = false;
+        try { run.compute_dt(0.5); } catch (const std::invalid_argument&) { rejected = true; }
+        // the library's driver on a second run, writing the reference's CSV files
         mhd_hip::Run bw(800, 4, 0.0, 1.0, 0.0, 4.0 / 800.0, 2.0, 0.4, mhd_hip::RiemannSolver::HLLD, bc);
         bw.init_state_device(MHD_CASE_BRIO_WU);
-        const mhd_drive_result r = bw.drive(MHD_CASE_BRIO_WU, "brio_wu", 0.1, "", 1, 0.1, false);
-        std::printf("ran %.17g %lld\n", t, r.steps);
+        const mhd_drive_result r = bw.drive(MHD_CASE_BRIO_WU, "brio_wu", t_end, argc > 1 ? argv[1] : "", 1, 0.1,
+                                            false);
+        std::vector<double> a(804 * 8 * 9), b(804 * 8 * 9);
+        run.download_to_aos(a.data());
+        bw.download_to_aos(b.data());
+        const bool same = std::memcmp(a.data(), b.data(), a.size() * sizeof(double)) == 0;
+        std::printf("ran %lld %lld %d %d %.17g\n", steps, r.steps, (int)same, (int)rejected, t);
     } catch (const std::runtime_error& e) {
         std::printf("error: %s\n", e.what());
     }
     return 0;
 }
-''')
+'''
+
+
+@pytest.fixture(scope="module")
+def cpp_front(tmp_path_factory):
+    """A program written against include/mhd_hip.hpp (the head/-shaped C++ front),
+    compiled with g++ -Wall -Wextra -Werror and linked against the library."""
+    import subprocess
+    d = tmp_path_factory.mktemp("front")
+    src = d / "front.cpp"
+    src.write_text(_FRONT_SRC)
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     libdir = os.path.join(root, "mhd_gpu_acc_amd")
-    exe = tmp_path / "front"
+    exe = d / "front"
     subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I" + os.path.join(root, "include"),
                     str(src), "-o", str(exe), "-L" + libdir, "-lmhd_hip", "-Wl,-rpath," + libdir,
                     "-Wl,-rpath,/opt/rocm/lib"], check=True)
-    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120).stdout
-    assert out.startswith("ran ") or "no HIP device" in out, out
-    if out.startswith("ran "):
-        assert out.split()[2] == "978", out  # the reference driver's Brio-Wu step count
+    return exe
+
+
+def test_cpp_front_compiles_links_and_reports(cpp_front, tmp_path):
+    """Without a GPU the run constructor throws the library's no-device error (no CPU
+    fallback); the device path is the gpu-marked test below."""
+    import subprocess
+    n = C.c_int()
+    m.lib().mhd_device_count(C.byref(n))
+    if n.value > 0:
+        pytest.skip("a HIP device is present: test_cpp_front_steps_brio_wu_on_the_gpu covers it")
+    out = subprocess.run([str(cpp_front), str(tmp_path)], capture_output=True, text=True, timeout=120).stdout
+    assert "no HIP device" in out, out
+
+
+@pytest.mark.gpu
+def test_cpp_front_steps_brio_wu_on_the_gpu(cpp_front, tmp_path):
+    """Run in a fresh process: the C++ front's compute_dt / advance_second_order loop
+    (the reference driver's, main_cpu.cpp:196-222) takes Brio-Wu to t = 0.1 in the
+    reference driver's 978 steps and ends in the state the library's driver reaches;
+    the driver's CSV files are byte for byte the reference driver's
+    (tests/golden/driver_reference.json); compute_dt with a CFL other than the run's
+    throws std::invalid_argument."""
+    import hashlib
+    import json
+    import subprocess
+    print("[gpu] C++ front: brio_wu hlld minmod to t=0.1", flush=True)
+    out = subprocess.run([str(cpp_front), str(tmp_path)], capture_output=True, text=True, timeout=120).stdout
+    assert out.startswith("ran "), out
+    f = out.split()
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "driver_reference.json")))["brio_wu"]
+    assert int(f[1]) == ref["steps"] == 978 and int(f[2]) == 978, out
+    assert f[3] == "1", "the front's loop and the library's driver end in different states"
+    assert f[4] == "1", "compute_dt with a different CFL was not rejected"
+    for name, meta in ref["files"].items():
+        ours = tmp_path / name.replace("_cpu_", "_gpu_")
+        assert hashlib.sha256(ours.read_bytes()).hexdigest() == meta["sha256"], name

@@ -226,6 +226,73 @@ def test_autograd_repeated_backward_and_tape_reuse():
             torch.autograd.grad(la, a)
 
 
+@pytest.mark.gpu
+def test_backward_of_an_older_graph_leaves_the_run_as_it_was():
+    """Forward A, forward B (the c_h ratchet carried on from A), backward A: the run's
+    c_h stays B's, and compute_dt and the next steps are what they would have been
+    without A's backward; A's gradient is that of a fresh forward + backward."""
+    import torch
+    from mhd_gpu_acc_amd.adjoint import run_steps, to_planes
+    say("[gpu] autograd: backward of an older graph, OT 32^2 HLLD van Leer 6 steps K=2")
+    case = small_case(32, 32, (0, 0, 0, 0))
+    U0 = perturbed_ot(case, 9)
+    with m.Run(cfg_of(case), 2, 1) as run:
+        run.ch = 0.0
+        a = to_planes(U0).requires_grad_(True)
+        la = (run_steps(a, run, 6, 2)[0] ** 2).sum()
+        b = to_planes(U0 * np.r_[1.002, 1, 1, 1, 1, 1, 1, 1, 1]).requires_grad_(True)
+        run_steps(b, run, 6, 2)  # no c_h reset: B continues A's ratchet
+        dt_b = run.compute_dt()  # ratchets c_h for the next step
+        ch_b, state_b = run.ch, run.get_state()
+        ga, = torch.autograd.grad(la, a)
+        assert run.ch == ch_b
+        assert run.compute_dt() == dt_b
+        assert_bitwise(run.get_state(), state_b, "state after the backward")
+        after = run.step_to(1.0, 3)
+        run.set_state(state_b)
+        run.ch = ch_b
+        control = run.step_to(1.0, 3)
+        assert_bitwise(after["dts"], control["dts"], "dt sequence after the backward")
+        run.ch = 0.0
+        a2 = to_planes(U0).requires_grad_(True)
+        ga2, = torch.autograd.grad((run_steps(a2, run, 6, 2)[0] ** 2).sum(), a2)
+        assert torch.equal(ga, ga2)
+
+
+@pytest.mark.gpu
+def test_autograd_is_stream_ordered_on_a_side_stream():
+    """The Function queues its work on torch's current stream: on a side stream, with no
+    host synchronisation between the producing op, the forward, the loss and the
+    backward, the gradient equals the default-stream one bit for bit, and the run is
+    handed back to its own stream."""
+    import torch
+    from mhd_gpu_acc_amd.adjoint import run_steps, to_planes
+    say("[gpu] autograd on a side stream, OT 40^2 HLLD van Leer 8 steps K=3")
+    case = small_case(40, 40, (0, 0, 0, 0))
+    U0 = perturbed_ot(case, 21)
+    scale = torch.tensor(np.r_[1.0, 1, 1, 1, 1, 1, 1, 1, 1], dtype=torch.float64, device="cuda")[:, None, None]
+
+    def grad_on(stream):
+        with torch.cuda.stream(stream):
+            base = to_planes(U0)
+            x = (base * scale).requires_grad_(True)  # produced on this stream just before
+            run.ch = 0.0
+            loss = (run_steps(x, run, 8, 3)[0] ** 2).sum()
+            g, = torch.autograd.grad(loss, x)
+            out = g.clone()
+        stream.synchronize()
+        return out
+
+    with m.Run(cfg_of(case), 2, 1) as run:
+        own = run.stream
+        g_default = grad_on(torch.cuda.current_stream())
+        side = torch.cuda.Stream()
+        g_side = grad_on(side)
+        assert run.stream == own
+    assert torch.equal(g_default, g_side)
+    assert torch.isfinite(g_side).all() and (g_side != 0).any()
+
+
 def _workload_gradient(run, U0, target, steps):
     from mhd_gpu_acc_amd.adjoint import from_planes, run_steps, to_planes
     run.ch = 0.0
