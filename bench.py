@@ -52,6 +52,8 @@ def main():
     ap.add_argument("--limiter", default="vanleer", choices=sorted(LIMITERS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--config-cpu-seconds", type=float, default=2.0,
+                    help="CPU sample per configuration for its CPU figure and parity columns")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form measurement")
     ap.add_argument("--batch-steps", type=int, default=40)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
@@ -147,7 +149,7 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint()
     if not args.no_configs:
-        out["configs"] = bench_configs()
+        out["configs"] = bench_configs(0.0 if args.no_cpu else args.config_cpu_seconds)
 
     if not args.no_cpu:
         import oracle
@@ -188,7 +190,8 @@ def main():
 _IC_CACHE = {}
 
 
-def _time_plain(case_name, nx, ny, solver, limiter, warmup, steps, t_end=None, params=None, cfl=None):
+def _time_plain(case_name, nx, ny, solver, limiter, warmup, steps, t_end=None, params=None, cfl=None,
+                cpu_seconds=0.0):
     """Plain step of one configuration: warm-up, then `steps` timed steps (or the run to
     t_end when steps is None), with the sweep kernels timed by HIP events.  A run the
     scheme cannot finish (the reference's blow-up stop, dt below 1e-8 t_end) reports
@@ -226,30 +229,72 @@ def _time_plain(case_name, nx, ny, solver, limiter, warmup, steps, t_end=None, p
         L.mhd_profile(run.handle, 0)
     k = max(nst.value, 1)
     ysweep = ms4[2] / k
-    return {"workload": f"{case_name} {nx}x{ny}, {solver.upper()}, {limiter}, CFL {cfg.cfl}", "steps": r["steps"],
-            "t": r["t"], "stop_code": r["code"], "value": r["steps"] * nx * ny / sec, "unit": "cell-updates/s",
-            "ms_per_step": 1e3 * sec / max(r["steps"], 1), "xsweep_ms": ms4[0] / k, "ysweep_ms": ysweep,
-            "ysweep_hbm_frac": BYTES_PER_CELL_SWEEP * nx * ny / (ysweep * 1e-3) / HBM_PEAK}
+    out = {"workload": f"{case_name} {nx}x{ny}, {solver.upper()}, {limiter}, CFL {cfg.cfl}", "steps": r["steps"],
+           "t": r["t"], "stop_code": r["code"], "value": r["steps"] * nx * ny / sec, "unit": "cell-updates/s",
+           "ms_per_step": 1e3 * sec / max(r["steps"], 1), "xsweep_ms": ms4[0] / k, "ysweep_ms": ysweep,
+           "ysweep_hbm_frac": BYTES_PER_CELL_SWEEP * nx * ny / (ysweep * 1e-3) / HBM_PEAK}
+    if cpu_seconds > 0:
+        out.update(_cpu_parity(U0, cfg, solver, limiter, cfg.t_end if t_end is None else t_end, cpu_seconds))
+    return out
 
 
-def bench_configs():
+def _cpu_parity(U0, cfg, solver, limiter, t_end, budget):
+    """The oracle (C++/OpenMP restatement of the reference CPU path) on the first steps of
+    the same run from the same initial state, about `budget` seconds of it, on the host's
+    cores; then the GPU over the same steps from the same state.  Returns the CPU
+    cell-updates/s and the parity columns: rel-L1 of the GPU state against the CPU's over
+    the nine interior fields, sum |gpu - cpu| / sum |cpu|, and the largest relative
+    deviation of the dt sequence."""
+    import numpy as np
+
+    import mhd_gpu_acc_amd as m
+    import oracle
+
+    case = oracle.Case(cfg.nx, cfg.ny, 2, cfg.x_min, cfg.x_max, cfg.y_min, cfg.y_max, cfg.cfl, t_end, cfg.gamma,
+                       tuple(cfg.bc.as_list()))
+    s, lim = SOLVERS[solver], LIMITERS[limiter]
+    _, t1 = oracle.run_loop_timed(U0, case, s, lim, 1)
+    k = int(max(1, min(10 ** 7, round(budget / max(t1, 1e-9)))))
+    res, secs = oracle.run_loop_timed(U0, case, s, lim, k)
+    n = res["nsteps"]
+    with m.Run(cfg, SOLVERS[solver], LIMITERS[limiter]) as run:
+        run.set_state(U0)
+        g = run.step_to(t_end, n)
+        Ug = run.get_state()
+    Uc = res["U"]
+    a, b = Ug[2:-2, 2:-2], Uc[2:-2, 2:-2]
+    den = float(np.abs(b).sum())
+    same_dt = g["steps"] == n
+    dev = float(np.max(np.abs(g["dts"] - res["dts"]) / res["dts"])) if same_dt and n else float("nan")
+    return {"cpu_value": n * cfg.nx * cfg.ny / secs, "cpu_cores": oracle.num_threads(), "cpu_kind": "port",
+            "cpu_sample": f"first {n} steps of the same run ({secs:.2f} s)",
+            "rel_l1": float(np.abs(a - b).sum()) / den if den > 0 else float("nan"),
+            "dt_max_rel_dev": dev,
+            "bitwise_vs_cpu": bool(same_dt and np.array_equal(Ug.view(np.uint64), Uc.view(np.uint64)))}
+
+
+def bench_configs(cpu_seconds: float = 2.0):
     """The other plain configurations: Orszag-Tang at 4096^2 and 1024^2 with each Riemann
     solver (van Leer, CFL 0.8, 5 warm-up + 100 timed steps), the 1024^2 HLLD run to
     t = 0.5 (CFL 0.4), the seeded shock-cloud 2048x1024 (HLLD, 200 steps) and Brio-Wu
-    800x4 (HLLD, minmod, CFL 0.8, to t = 0.1)."""
+    800x4 (HLLD, minmod, CFL 0.8, to t = 0.1).  Each one also times the oracle on about
+    cpu_seconds of its first steps on the host's cores and reports the GPU's parity over
+    those steps (cpu_value, rel_l1, dt_max_rel_dev; 0 skips that leg)."""
     import mhd_gpu_acc_amd as m
 
+    c = cpu_seconds
     res = []
     for n in (4096, 1024):
         for s in SOLVERS:
-            res.append(_time_plain("orszag_tang_unit", n, n, s, "vanleer", 5, 100))
+            res.append(_time_plain("orszag_tang_unit", n, n, s, "vanleer", 5, 100, cpu_seconds=c))
     # the accuracy run to t = 0.5: HLLD reaches it at the reference's own Orszag-Tang CFL
     # of 0.4; at 0.8 its run (like the reference's) stops at the blow-up guard near t = 0.14
-    res.append(_time_plain("orszag_tang_unit", 1024, 1024, "hlld", "vanleer", 0, None, t_end=0.5, cfl=0.4))
+    res.append(_time_plain("orszag_tang_unit", 1024, 1024, "hlld", "vanleer", 0, None, t_end=0.5, cfl=0.4,
+                           cpu_seconds=c))
     p = m.ICParams()
     p.seed = 20261203
-    res.append(_time_plain("shock_cloud", 2048, 1024, "hlld", "vanleer", 5, 200, params=p))
-    res.append(_time_plain("brio_wu_transmissive", 800, 4, "hlld", "minmod", 0, None))
+    res.append(_time_plain("shock_cloud", 2048, 1024, "hlld", "vanleer", 5, 200, params=p, cpu_seconds=c))
+    res.append(_time_plain("brio_wu_transmissive", 800, 4, "hlld", "minmod", 0, None, cpu_seconds=c))
     _IC_CACHE.clear()
     return res
 

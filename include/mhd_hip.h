@@ -213,9 +213,10 @@ int mhd_batch_plan_ot_family(uint64_t seed, int n, int* sizes, mhd_ic_params* pa
  * compute_dt + advance_second_order.  dt and c_h are frozen at the recorded
  * values and every branch (Riemann-solver region and fallbacks, limiter
  * selections, physical-state guards) at the one the recorded trajectory takes.
- * Supported: every Riemann solver (HLL, HLLC, HLLD, FORCE) and limiter, periodic /
- * transmissive / reflecting sides (MHD_ERR_INVALID for Dirichlet).  The gradient is with respect to
- * every value of U_0, ghost cells included. */
+ * Supported: every Riemann solver (HLL, HLLC, HLLD, FORCE) and limiter, and every boundary
+ * type; through a Dirichlet side the ghost cells copied from the previous grid carry
+ * their adjoint back to it (boundary_cpu.hpp:12-32, solver_cpu.cpp:396-398,434-436).  The
+ * gradient is with respect to every value of U_0, ghost cells included. */
 typedef struct mhd_tape mhd_tape;
 /* Device memory: ceil(nsteps/K) checkpoints, 2K+1 segment grids, 2 adjoint grids. */
 int mhd_tape_create(mhd_run* run, int nsteps, int K, mhd_tape** out);

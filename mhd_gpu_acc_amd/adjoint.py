@@ -10,7 +10,8 @@ checkpoint, then reversed.  dt and c_h are frozen at the values the forward
 recorded, and every branch (Riemann regions and fallbacks, limiter selections,
 physical-state guards) at the one the forward took; the gradient is with respect
 to every value of U0, ghost cells included.  Every Riemann solver (HLL, HLLC,
-HLLD, FORCE) and limiter; periodic / transmissive / reflecting sides.
+HLLD, FORCE) and limiter, and every boundary type (periodic, transmissive,
+reflecting, Dirichlet).
 
 Nothing here computes on the host: both directions are kernels of libmhd_hip.so.
 """

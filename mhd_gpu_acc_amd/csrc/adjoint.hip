@@ -39,8 +39,19 @@ template <bool XDIR, bool BUF> struct LamPlanes {
     MHD_DI Cons operator()(int i, int j) const { return p.load(j * tnx + i); }
 };
 
+// Knob (compile time): MHD_ADJ_WAVES_PER_EU asks the compiler for that many resident
+// waves per SIMD (0: its own choice).
+#ifndef MHD_ADJ_WAVES_PER_EU
+#define MHD_ADJ_WAVES_PER_EU 0
+#endif
+#if MHD_ADJ_WAVES_PER_EU > 0
+#define MHD_ADJ_ATTR __attribute__((amdgpu_waves_per_eu(MHD_ADJ_WAVES_PER_EU)))
+#else
+#define MHD_ADJ_ATTR
+#endif
+
 template <int SOLVER, int LIMITER, bool XDIR, int NL, int WG, bool BUF>
-__global__ __launch_bounds__(WG) void adj_sweep_kernel(const double* __restrict__ U, const double* __restrict__ Lin,
+__global__ __launch_bounds__(WG) MHD_ADJ_ATTR void adj_sweep_kernel(const double* __restrict__ U, const double* __restrict__ Lin,
                                                        double* __restrict__ Lout, Geom g, Phys ph, double dtd,
                                                        double hdtd, double psi_scale, Bc4 bc, int ntiles) {
     constexpr int LT = WG / NL, NOUT = LT - 6;
@@ -142,6 +153,24 @@ __global__ __launch_bounds__(WG) void adj_sweep_kernel(const double* __restrict_
         std::conditional_t<BUF, BufPlanes<XDIR>, Planes<XDIR, double>>(Lout, g.ps).store(kc, out);
 }
 
+// Dirichlet transpose (dirichlet_gather) over every ghost cell of the previous grid's
+// adjoint LP, from the filled grid's adjoint LG.  keep_all: every target keeps the value
+// the preceding reverse pass wrote (after the x pass); otherwise only the ghost rows of
+// the interior columns do (after the y pass, which writes no other ghost cell of U*'s
+// adjoint) and the rest are set, so nothing of an earlier step survives there.
+__global__ __launch_bounds__(256) void adj_dirichlet_kernel(const double* __restrict__ LG, double* __restrict__ LP,
+                                                            Geom g, Bc4 bc, int keep_all) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ghost_count(g)) return;
+    int ti, tj;
+    ghost_cell(g, t, ti, tj);
+    const LamPlanes<true, false> lg(LG, g.ps, g.tnx);
+    const Planes<true, double> lp(LP, g.ps);
+    const int k = tj * g.tnx + ti;
+    const bool keep = keep_all || (ti >= NG && ti < NG + g.nx);
+    lp.store(k, dirichlet_gather(lg, g, bc.s, ti, tj, keep ? lp.load(k) : cons_zero()));
+}
+
 // Tile shapes (compile-time knobs): workgroup width and, for the y pass, columns
 // per tile.  At one wave per SIMD (the kernels hold ~410 registers) a CU runs
 // whatever workgroups fit, and small workgroups keep a barrier from idling the
@@ -170,12 +199,25 @@ static hipError_t launch_adjoint_sla(const AdjArgs& a) {
                        a.lam_tmp, g, ph, dtdy, 0.5 * dtdy, damping ? a.damp : 1.0, bc, 0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    const bool dirichlet = any_dirichlet(bc.s);
+    const int nghost = ghost_count(g);
+    if (dirichlet) {  // U_next's ghosts copied from U* -> U*'s adjoint
+        hipLaunchKernelGGL(adj_dirichlet_kernel, dim3((nghost + 255) / 256), dim3(256), 0, a.stream, a.lam,
+                           a.lam_tmp, g, bc, 0);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     const long long flat = (long long)g.tnx * g.ny;
     const int ntiles = (int)((flat + NOUTX - 1) / NOUTX);
     const int nzero = (2 * NG * g.tnx + XWG - 1) / XWG;
     hipLaunchKernelGGL((adj_sweep_kernel<S, L, true, 1, XWG, BUF>), dim3((unsigned)(ntiles + nzero)), dim3(XWG), 0,
                        a.stream, a.Uold, a.lam_tmp, a.lam, g, ph, dtdx, 0.5 * dtdx, 1.0, bc, ntiles);
-    return hipGetLastError();
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (dirichlet) {  // U*'s ghosts copied from U -> U's adjoint
+        hipLaunchKernelGGL(adj_dirichlet_kernel, dim3((nghost + 255) / 256), dim3(256), 0, a.stream, a.lam_tmp,
+                           a.lam, g, bc, 1);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // Buffer-resource addressing for grids of at most 4 GiB, as the forward sweeps.

@@ -75,10 +75,11 @@ MHD_DI Cons adj_cell(const Cons& u, const Prim& w, const Prim& pprev, const Prim
     return out;
 }
 
-// Adjoint of the interior cell (i, j) of a grid whose ghost cells were filled from
-// its own interior (no Dirichlet side): the cell's own adjoint (psi scaled by the
-// damping factor that followed the fill, 1 when none) plus the adjoints of every
-// ghost cell ghost_source() maps onto it, with the reflections' sign flips.
+// Adjoint of the interior cell (i, j) of a grid after its ghost fill: the cell's own
+// adjoint (psi scaled by the damping factor that followed the fill, 1 when none) plus
+// the adjoints of every ghost cell ghost_source() maps onto it, with the reflections'
+// sign flips.  (Ghost cells copied from the previous grid through a Dirichlet side are
+// transposed by dirichlet_gather.)
 // lam(i, j) returns the adjoint at a cell in the rotated frame of the sweep.
 template <bool XDIR, class Lam>
 MHD_DI Cons gather_adj(const Lam& lam, const Geom& g, const int* bc, int i, int j, double psi_scale) {
@@ -105,5 +106,34 @@ MHD_DI Cons gather_adj(const Lam& lam, const Geom& g, const int* bc, int i, int 
         }
     return v;
 }
+
+// Adjoint of the ghost cells that a ghost fill copies from the previous grid through a
+// Dirichlet side (ghost_source(...).prev: copy_ghost_cells followed by a side the fill
+// leaves alone, head/cpu/boundary_cpu.hpp:12-32, solver_cpu.cpp:396-398,434-436).  Such a
+// source is always a ghost cell of the previous grid.  The adjoint of the previous grid
+// at its ghost cell (ti, tj) is v plus the adjoint of every ghost cell of the filled
+// grid whose value was copied from there, with the reflections' sign flips; lam(i, j)
+// returns the filled grid's adjoint in the reference frame (x frame).  The candidates
+// are those of gather_adj, each once, in a fixed order.
+template <class Lam>
+MHD_DI Cons dirichlet_gather(const Lam& lam, const Geom& g, const int* bc, int ti, int tj, Cons v) {
+    const int gx[5] = {ti, 0, 1, NG + g.nx, NG + g.nx + 1};
+    const int gy[5] = {tj, 0, 1, NG + g.ny, NG + g.ny + 1};
+    for (int a = 0; a < 5; ++a) {
+        if (a > 0 && gx[a] == ti) continue;
+        for (int b = 0; b < 5; ++b) {
+            if (b > 0 && gy[b] == tj) continue;
+            const Src s = ghost_source(gx[a], gy[b], g, bc);
+            if (!s.prev || s.i != ti || s.j != tj) continue;
+            Cons w = lam(gx[a], gy[b]);
+            if (s.flip_x) { w.mn = -w.mn; w.bn = -w.bn; }
+            if (s.flip_y) { w.mt = -w.mt; w.bt = -w.bt; }
+            acc(v, w);
+        }
+    }
+    return v;
+}
+
+MHD_DI bool any_dirichlet(const int* bc) { return bc[0] == 3 || bc[1] == 3 || bc[2] == 3 || bc[3] == 3; }
 
 }  // namespace mhd

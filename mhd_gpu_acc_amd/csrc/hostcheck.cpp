@@ -98,6 +98,23 @@ void reverse_sweep(const double* U, const double* Lin, double* Lout, const Geom&
     }
 }
 
+// The Dirichlet transpose over every ghost cell of the previous grid's adjoint LP from
+// the filled grid's adjoint LG (dirichlet_gather), as adj_dirichlet_kernel: the cells
+// the preceding reverse pass wrote keep their value and gain the copies' adjoints
+// (keep_all: every ghost cell; otherwise the ghost rows of the interior columns), the
+// others are set to the copies' adjoints alone.
+void dirichlet_pass(const double* LG, double* LP, const Geom& g, const int* bc, bool keep_all) {
+    const HostLam lg{LG, g.tnx, true};
+    for (int t = 0; t < ghost_count(g); ++t) {
+        int ti, tj;
+        ghost_cell(g, t, ti, tj);
+        double* p = LP + 9 * ((size_t)tj * g.tnx + ti);
+        const bool keep = keep_all || (ti >= NG && ti < NG + g.nx);
+        const Cons v = dirichlet_gather(lg, g, bc, ti, tj, keep ? rot(p, true) : cons_zero());
+        unrot(v, true, p);
+    }
+}
+
 template <int S, int L>
 void reverse_step(const double* Uold, const double* Utmp, double* lam, const Geom& g, double dx, double dy, double dt,
                   double ch, double cr, double gamma, const int* bc) {
@@ -107,11 +124,13 @@ void reverse_step(const double* Uold, const double* Utmp, double* lam, const Geo
     const bool damping = !(damp >= 1.0);
     std::vector<double> B(9 * (size_t)g.tnx * g.tny, 0.0);
     reverse_sweep<S, L, false>(Utmp, lam, B.data(), g, ph, dtdy, 0.5 * dtdy, damping ? damp : 1.0, bc);
+    if (any_dirichlet(bc)) dirichlet_pass(lam, B.data(), g, bc, false);
     reverse_sweep<S, L, true>(Uold, B.data(), lam, g, ph, dtdx, 0.5 * dtdx, 1.0, bc);
     for (int j = 0; j < g.tny; ++j) {
         if (j >= NG && j < NG + g.ny) continue;
         std::memset(lam + 9 * (size_t)j * g.tnx, 0, 9 * (size_t)g.tnx * sizeof(double));
     }
+    if (any_dirichlet(bc)) dirichlet_pass(B.data(), lam, g, bc, true);
 }
 
 }  // namespace
@@ -215,7 +234,7 @@ int hc_adjoint_step(const double* Uold, const double* Utmp, int nx, int ny, doub
                     double y1, double dt, double ch, double cr, double gamma, int solver, int limiter, const int* bc,
                     double* lam) {
     for (int k = 0; k < 4; ++k)
-        if (bc[k] < 0 || bc[k] > 2) return 1;
+        if (bc[k] < 0 || bc[k] > 3) return 1;
     const Geom g{nx, ny, nx + 2 * NG, ny + 2 * NG, 0};
     const double dx = (x1 - x0) / static_cast<double>(nx), dy = (y1 - y0) / static_cast<double>(ny);
     switch (solver * 2 + limiter) {

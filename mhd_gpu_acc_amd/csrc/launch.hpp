@@ -47,7 +47,7 @@ struct AdjArgs {
     double gamma, ch;
     double damp;             // the step's psi damping factor
     int solver, limiter;     // solver 0 (HLL) or 2 (HLLD)
-    int bc[4];               // no Dirichlet side
+    int bc[4];
     bool addr_buf;           // address through buffer resources where the grid allows (else flat)
     hipStream_t stream;
 };

@@ -371,10 +371,9 @@ MHD_DI Src ghost_source(int gi, int gj, const Geom& g, const int* bc) {
 // (row-major), then the 2NG ghost rows of tnx cells.
 MHD_DI int ghost_count(const Geom& g) { return 2 * NG * g.ny + 2 * NG * g.tnx; }
 
-MHD_DI void ghost_fill(double* __restrict__ U, const double* __restrict__ prev, const Geom& g, const int* bc,
-                       const double* __restrict__ frame, int t) {
+// Position (gi, gj) of ghost cell number t (0 <= t < ghost_count(g)).
+MHD_DI void ghost_cell(const Geom& g, int t, int& gi, int& gj) {
     const int n_side = 2 * NG * g.ny;
-    int gi, gj;
     if (t < n_side) {
         // four consecutive lanes per row: columns 0, 1, ie, ie+1
         const int c = t % (2 * NG);
@@ -386,6 +385,12 @@ MHD_DI void ghost_fill(double* __restrict__ U, const double* __restrict__ prev, 
         gi = q % g.tnx;
         gj = r < NG ? r : g.ny + r;
     }
+}
+
+MHD_DI void ghost_fill(double* __restrict__ U, const double* __restrict__ prev, const Geom& g, const int* bc,
+                       const double* __restrict__ frame, int t) {
+    int gi, gj;
+    ghost_cell(g, t, gi, gj);
     const Src s = ghost_source(gi, gj, g, bc);
     const double* src = s.prev ? prev : U;
     const long long ks = (long long)s.j * g.tnx + s.i;

@@ -92,8 +92,6 @@ int download(mhd_run* r, const double* src, int layout, double* dst) {
 
 int check_adjoint_run(const mhd_run* r) {
     if (r->d.solver < MHD_HLL || r->d.solver > MHD_FORCE) return fail(MHD_ERR_INVALID, "adjoint: unknown Riemann solver");
-    for (int k = 0; k < 4; ++k)
-        if (r->d.bc[k] == MHD_BC_DIRICHLET) return fail(MHD_ERR_INVALID, "adjoint: Dirichlet boundaries are not supported");
     return MHD_OK;
 }
 
@@ -171,7 +169,8 @@ int mhd_tape_create(mhd_run* r, int nsteps, int K, mhd_tape** out) {
     if (!rc) {
         hipError_t e = hipMalloc(&t->count, sizeof(unsigned int));
         if (e == hipSuccess) e = hipHostMalloc(&t->host_count, sizeof(unsigned int), hipHostMallocDefault);
-        // the y pass never writes the ghost columns of the U* adjoint: they stay zero
+        // the y pass never writes the ghost columns of the U* adjoint: they stay zero (with a
+        // Dirichlet side the transpose of the copies from U* sets them every step)
         if (e == hipSuccess) e = hipMemsetAsync(t->lam[1], 0, grid_bytes(r), r->stream());
         if (e == hipSuccess) e = hipStreamSynchronize(r->stream());
         if (e != hipSuccess) rc = mhd::hip_fail(e, "tape allocation");

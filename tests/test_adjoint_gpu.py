@@ -43,7 +43,8 @@ def cfg_of(case):
 STEP_CASES = [(13, 10, (0, 0, 0, 0), 2, 1), (37, 29, (2, 1, 0, 0), 2, 0), (64, 40, (1, 1, 2, 2), 0, 1),
               (251, 7, (0, 0, 1, 2), 2, 1), (5, 90, (2, 2, 0, 0), 0, 0), (70, 70, (0, 0, 0, 0), 2, 1),
               (41, 23, (0, 0, 0, 0), 1, 1), (19, 66, (2, 1, 0, 0), 1, 0), (70, 31, (1, 1, 2, 2), 3, 1),
-              (33, 47, (0, 0, 1, 2), 3, 0)]
+              (33, 47, (0, 0, 1, 2), 3, 0), (29, 23, (3, 3, 3, 3), 2, 1), (66, 9, (3, 1, 0, 0), 0, 1),
+              (17, 70, (2, 3, 1, 3), 1, 1), (70, 40, (0, 0, 3, 2), 3, 1)]
 
 
 @pytest.mark.gpu
@@ -94,7 +95,8 @@ def _oracle_chain(case, U0, nsteps, solver, limiter):
 @pytest.mark.gpu
 @pytest.mark.parametrize("solver,limiter,bc,K", [(2, 1, (0, 0, 0, 0), 5), (0, 0, (1, 2, 2, 1), 12),
                                                  (2, 0, (2, 2, 0, 0), 1), (1, 1, (0, 0, 0, 0), 4),
-                                                 (3, 0, (1, 1, 2, 2), 6)])
+                                                 (3, 0, (1, 1, 2, 2), 6), (2, 1, (3, 1, 3, 2), 5),
+                                                 (2, 0, (3, 3, 3, 3), 4)])
 def test_tape_matches_host_chain_and_tangent(solver, limiter, bc, K):
     from mhd_gpu_acc_amd.adjoint import Tape
     say(f"[gpu] tape 40x36 solver={solver} limiter={limiter} bc={bc} K={K}")
@@ -125,14 +127,14 @@ def test_tape_matches_host_chain_and_tangent(solver, limiter, bc, K):
 
 
 @pytest.mark.gpu
-def test_tape_rejects_unsupported():
+def test_tape_argument_checks():
     from mhd_gpu_acc_amd.adjoint import Tape
-    case = small_case(8, 8, (3, 0, 0, 0))
-    with m.Run(cfg_of(case), 2, 1) as run:
-        with pytest.raises(m.MHDError):
-            Tape(run, 4, 2)
-    with m.Run(cfg_of(small_case(8, 8, (0, 0, 0, 0))), 1, 1) as run:  # HLLC is supported
+    with m.Run(cfg_of(small_case(8, 8, (3, 0, 0, 0))), 2, 1) as run:  # Dirichlet sides are supported
         Tape(run, 4, 2).close()
+        with pytest.raises(m.MHDError):
+            Tape(run, 0, 2)
+        with pytest.raises(m.MHDError):
+            Tape(run, 4, 0)
 
 
 def _loss_from_planes(UN, target):

@@ -203,7 +203,9 @@ def perturbed_ot(case, seed):
     return U
 
 
-BCS = [(0, 0, 0, 0), (1, 1, 1, 1), (2, 2, 2, 2), (0, 0, 2, 1), (2, 1, 0, 0), (1, 2, 2, 0)]
+BCS = [(0, 0, 0, 0), (1, 1, 1, 1), (2, 2, 2, 2), (0, 0, 2, 1), (2, 1, 0, 0), (1, 2, 2, 0),
+       # Dirichlet sides: ghosts copied from the previous grid (alone, beside each other type)
+       (3, 3, 3, 3), (3, 1, 0, 0), (0, 0, 3, 2), (2, 3, 1, 3), (1, 3, 3, 1)]
 
 
 @pytest.mark.parametrize("bc", BCS)
@@ -238,7 +240,7 @@ def test_step_adjoint_ghost_rows_and_corners_zero():
 
 
 @pytest.mark.parametrize("solver,limiter,bc", [(2, 1, (0, 0, 0, 0)), (0, 0, (1, 2, 0, 0)), (1, 1, (0, 0, 2, 2)),
-                                               (3, 0, (2, 1, 0, 0))])
+                                               (3, 0, (2, 1, 0, 0)), (2, 1, (3, 1, 3, 0)), (1, 0, (3, 3, 3, 3))])
 def test_multistep_adjoint_dot(solver, limiter, bc):
     """Five steps with the c_h ratchet and damping: reverse steps chained over the
     stored states against the tangent run with the recorded dt / c_h."""
@@ -266,12 +268,29 @@ def test_multistep_adjoint_dot(solver, limiter, bc):
 
 
 def test_unsupported_configurations_rejected():
-    case = small_case(6, 6, (3, 0, 0, 0))
     U = perturbed_ot(small_case(6, 6, (0, 0, 0, 0)), 1)
-    with pytest.raises(ValueError):
-        hc.adjoint_step(U, U, case, 1e-3, 1.0, 2, 1, U)
     with pytest.raises(ValueError):  # no solver 4
         hc.adjoint_step(U, U, small_case(6, 6, (0, 0, 0, 0)), 1e-3, 1.0, 4, 1, U)
+    with pytest.raises(ValueError):  # no boundary type 4
+        hc.adjoint_step(U, U, small_case(6, 6, (4, 0, 0, 0)), 1e-3, 1.0, 2, 1, U)
+
+
+def test_dirichlet_ghost_adjoint_reaches_the_previous_grid():
+    """Through Dirichlet sides the input's ghost rows and columns feed the next state
+    (their values are copied into U* and on into U_next): their adjoint is not zero,
+    and a perturbation of one of them moves the tangent as the adjoint says."""
+    case = small_case(9, 7, (3, 3, 3, 3))
+    U = perturbed_ot(case, 4)
+    rc, dt, ch = oracle.compute_dt(U, case, case.cfl, 0.0)
+    Utmp, _ = oracle.advance(U, case, dt, ch, 2, 1)
+    lam = np.random.default_rng(2).normal(0, 1, U.shape)
+    A = hc.adjoint_step(U, Utmp, case, dt, ch, 2, 1, lam)
+    assert np.any(A[:2] != 0) and np.any(A[-2:] != 0) and np.any(A[:, :2] != 0)
+    for (j, i, f) in [(0, 5, 0), (1, 1, 7), (10, 3, 4), (5, 11, 2), (0, 0, 8)]:
+        e = np.zeros_like(U)
+        e[j, i, f] = 1.0
+        _, dU1 = oracle.tangent_run(U, e, case, [dt], [ch], 2, 1)
+        close(np.sum(lam * dU1), A[j, i, f], np.sum(np.abs(lam * dU1)) + abs(A[j, i, f]))
 
 
 def test_frozen_branch_differences_match_tangent():
