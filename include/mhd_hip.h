@@ -167,6 +167,22 @@ int mhd_apply_boundary(mhd_run* run);
 int mhd_compute_dt_grid(mhd_run* run, const double* U, double* dt);
 int mhd_advance_grids(mhd_run* run, const double* Uold, double* Utmp, double* Unew, double dt);
 
+/* mhd_advance_grids for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  It takes the run description and c_h directly instead of a run handle,
+ * and calls no hipMalloc, hipFree or other allocating routine: the step's scratch
+ * (CFL accumulator and the 4 (nx + ny)-double psi frame) lives in a fixed device array
+ * of MHD_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends, so
+ * nx + ny <= MHD_NOALLOC_MAX_NXNY (MHD_ERR_INVALID beyond).  Asynchronous on
+ * desc->stream; max_speed (nullable) receives the CFL maximum of Unew, max over the
+ * interior of fmax(signal speed, ch) -- the next compute_dt's c_h -- and then the call
+ * synchronises the stream.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams, other threads) race. */
+#define MHD_NOALLOC_SCRATCH_BYTES (1 << 20)
+#define MHD_NOALLOC_MAX_NXNY ((MHD_NOALLOC_SCRATCH_BYTES - 256) / 32)
+int mhd_advance_grids_noalloc(const mhd_run_desc* desc, double ch, const double* Uold, double* Utmp, double* Unew,
+                              double dt, double* max_speed);
+
 /* Synchronise the run's stream (errors from asynchronous launches surface here). */
 int mhd_sync(mhd_run* run);
 
