@@ -141,7 +141,10 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": (float(traffic["bytes_per_launch_" + dom]) if traffic and
                                  ("bytes_per_launch_" + dom) in traffic else None),
-                     "algorithmic_bytes_per_launch": BYTES_PER_CELL_SWEEP * cells},
+                     "algorithmic_bytes_per_launch": BYTES_PER_CELL_SWEEP * cells,
+                     "traffic_source": (f"profiles/pmc_ot{n}_{args.solver}_{args.limiter}.txt (rocprofv3 --pmc "
+                                        "FETCH_SIZE / WRITE_SIZE runs of this kernel, copy-calibrated), not "
+                                        "measured in this run") if traffic else None},
     }
 
     if not args.no_batch:

@@ -8,7 +8,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmhd_hip.so")
+# MHD_HIP_LIB: another build of the same library (a measurement knob, e.g. one built with
+# EXTRA=-DMHD_FP_CONTRACT_FAST=1); the default is the in-tree build.
+LIB_PATH = os.environ.get("MHD_HIP_LIB") or os.path.join(HERE, "libmhd_hip.so")
 
 MHD_OK = 0
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "non-positive maximum wave speed",

@@ -28,7 +28,7 @@
 #include "launch.hpp"
 #include "sweeps.hpp"
 
-#pragma clang fp contract(off)
+MHD_FP_CONTRACT
 
 namespace mhd {
 

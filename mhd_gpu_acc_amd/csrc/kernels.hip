@@ -22,12 +22,23 @@
 
 #include "sweeps.hpp"
 
-#pragma clang fp contract(off)
+MHD_FP_CONTRACT
 
 namespace mhd {
 
+// Knob (compile time): MHD_X_WAVES_PER_EU asks the compiler for that many resident waves
+// per SIMD in the x sweep (0: its own choice, 2 for HLLD).
+#ifndef MHD_X_WAVES_PER_EU
+#define MHD_X_WAVES_PER_EU 0
+#endif
+#if MHD_X_WAVES_PER_EU > 0
+#define MHD_X_ATTR __attribute__((amdgpu_waves_per_eu(MHD_X_WAVES_PER_EU)))
+#else
+#define MHD_X_ATTR
+#endif
+
 template <int SOLVER, int LIMITER, int WG, bool BUF>
-__global__ __launch_bounds__(WG) void xsweep_kernel(const double* __restrict__ Uo, double* __restrict__ Ut, Geom g,
+__global__ __launch_bounds__(WG) MHD_X_ATTR void xsweep_kernel(const double* __restrict__ Uo, double* __restrict__ Ut, Geom g,
                                                     Phys ph, double dtd, double hdtd,
                                                     unsigned long long* __restrict__ acc, int nseg) {
     __shared__ XLds<WG> L;

@@ -10,7 +10,7 @@
 
 #include "physics.hpp"
 
-#pragma clang fp contract(off)
+MHD_FP_CONTRACT
 
 namespace mhd {
 
