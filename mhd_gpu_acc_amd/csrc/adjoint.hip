@@ -50,6 +50,35 @@ template <bool XDIR, bool BUF> struct LamPlanes {
 #define MHD_ADJ_ATTR
 #endif
 
+// A lane's record of its reconstruction (ReconRecord), kept from step 2 to step 4 so
+// that the reconstruction adjoint does not form the reconstruction a second time.  Only
+// the lane that wrote a slot reads it: no barrier involved.
+template <int WG> struct RecLds {
+    double s[9][WG];
+    Lds9<WG> ul, ur;
+    unsigned char flags[WG];
+    MHD_DI void put(int t, const ReconRecord& r) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) s[k][t] = r.s[k];
+        ul.put(t, r.ul);
+        ur.put(t, r.ur);
+        flags[t] = (unsigned char)((r.pl ? 1 : 0) | (r.pr ? 2 : 0) | (r.pcl ? 4 : 0) | (r.pcr ? 8 : 0));
+    }
+    MHD_DI ReconRecord get(int t) const {
+        ReconRecord r;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) r.s[k] = s[k][t];
+        r.ul = ul.get(t);
+        r.ur = ur.get(t);
+        const unsigned f = flags[t];
+        r.pl = f & 1;
+        r.pr = f & 2;
+        r.pcl = f & 4;
+        r.pcr = f & 8;
+        return r;
+    }
+};
+
 template <int SOLVER, int LIMITER, bool XDIR, int NL, int WG, bool BUF>
 __global__ __launch_bounds__(WG) MHD_ADJ_ATTR void adj_sweep_kernel(const double* __restrict__ U, const double* __restrict__ Lin,
                                                        double* __restrict__ Lout, Geom g, Phys ph, double dtd,
@@ -59,6 +88,7 @@ __global__ __launch_bounds__(WG) MHD_ADJ_ATTR void adj_sweep_kernel(const double
     __shared__ Lds9<WG> sA;      // lower interface state (cons), then the bar of W[c-1]
     __shared__ Lds9<WG, 10> sB;  // its primitive, then the bar of W[c+1]
     __shared__ Lds9<WG> sG;      // output adjoint g, then the bar of the lower interface state
+    __shared__ RecLds<WG> sR;    // each lane's own forward record of its reconstruction
     const int t = threadIdx.x;
 
     if (XDIR && (int)blockIdx.x >= ntiles) {
@@ -113,7 +143,9 @@ __global__ __launch_bounds__(WG) MHD_ADJ_ATTR void adj_sweep_kernel(const double
     // 2. interface states and the output adjoint of the cell
     Face lo, hi;
     if (fl.rec) {
-        reconstruct<LIMITER>(sW.getp(tm), w, sW.getp(tp), hdtd, ph, lo, hi);
+        ReconRecord rec;
+        reconstruct<LIMITER>(sW.getp(tm), w, sW.getp(tp), hdtd, ph, lo, hi, &rec);
+        sR.put(t, rec);
     } else {
         lo.u = u; lo.w = w; hi = lo;
     }
@@ -137,7 +169,7 @@ __global__ __launch_bounds__(WG) MHD_ADJ_ATTR void adj_sweep_kernel(const double
     Prim wmb = prim_zero(), wcb = prim_zero(), wpb = prim_zero();
     if (fl.rec) {
         const Cons lob = pos > 0 ? sG.get(tm) : cons_zero();
-        reconstruct_adj<LIMITER>(sW.getp(tm), w, sW.getp(tp), hdtd, ph, lob, hib, wmb, wcb, wpb);
+        reconstruct_adj_rec<LIMITER>(sW.getp(tm), w, sW.getp(tp), hdtd, ph, sR.get(t), lob, hib, wmb, wcb, wpb);
     }
     sA.put(t, Cons{wmb.r, wmb.un, wmb.ut, wmb.uz, wmb.bn, wmb.bt, wmb.bz, wmb.p, wmb.psi});
     sB.put(t, wpb);

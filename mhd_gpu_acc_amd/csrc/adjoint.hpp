@@ -904,47 +904,29 @@ template <int LIMITER> MHD_DI void limit_adj(double a, double b, double sb, doub
 
 // Adjoint of reconstruct<LIMITER>(wm, wc, wp) given the bars of the two interface
 // states as conserved vectors (the bar of a face's primitive folded in by
-// to_prim_adj).  Accumulates into the nine primitive slots of wmb, wcb, wpb.
+// to_prim_adj), from reconstruct()'s record of its forward values.  Accumulates into
+// the nine primitive slots of wmb, wcb, wpb.
 template <int LIMITER>
-MHD_DI void reconstruct_adj(const Prim& wm, const Prim& wc, const Prim& wp, double hdtd, const Phys& ph,
-                            const Cons& lob, const Cons& hib, Prim& wmb, Prim& wcb, Prim& wpb) {
+MHD_DI void reconstruct_adj_rec(const Prim& wm, const Prim& wc, const Prim& wp, double hdtd, const Phys& ph,
+                                const ReconRecord& rec, const Cons& lob, const Cons& hib, Prim& wmb, Prim& wcb,
+                                Prim& wpb) {
+    const bool pl = rec.pl, pr = rec.pr, pcl = rec.pcl, pcr = rec.pcr;
+    // the guarded interface primitives, bit for bit as reconstruct() formed them
     Prim wl, wr;
-    double sl_[9];
-#define MHD_SLOPE(k, f)                                            \
-    {                                                              \
-        const double s = limit<LIMITER>(wc.f - wm.f, wp.f - wc.f); \
-        sl_[k] = s;                                                \
-        wl.f = wc.f - 0.5 * s;                                     \
-        wr.f = wc.f + 0.5 * s;                                     \
+#define MHD_SIDES(k, f)                                \
+    {                                                  \
+        wl.f = pl ? wc.f - 0.5 * rec.s[k] : wc.f;      \
+        wr.f = pr ? wc.f + 0.5 * rec.s[k] : wc.f;      \
     }
-    MHD_SLOPE(0, r) MHD_SLOPE(1, un) MHD_SLOPE(2, ut) MHD_SLOPE(3, uz) MHD_SLOPE(4, bn) MHD_SLOPE(5, bt)
-    MHD_SLOPE(6, bz) MHD_SLOPE(7, p) MHD_SLOPE(8, psi)
-#undef MHD_SLOPE
-    (void)sl_;
+    MHD_SIDES(0, r) MHD_SIDES(1, un) MHD_SIDES(2, ut) MHD_SIDES(3, uz) MHD_SIDES(4, bn) MHD_SIDES(5, bt)
+    MHD_SIDES(6, bz) MHD_SIDES(7, p) MHD_SIDES(8, psi)
+#undef MHD_SIDES
     wl.b2 = 0.0;
     wr.b2 = 0.0;
-    const bool pl = is_physical(wl);
-    const bool pr = is_physical(wr);
-    if (!pl) wl = wc;
-    if (!pr) wr = wc;
-    const Cons ul = to_cons(wl, ph);
-    const Cons ur = to_cons(wr, ph);
+    const Cons& ul = rec.ul;
+    const Cons& ur = rec.ur;
     const Prim vl = to_prim(ul, ph);
     const Prim vr = to_prim(ur, ph);
-    const Cons fl = flux(ul, vl, ph);
-    const Cons fr = flux(ur, vr, ph);
-    Cons cl, cr;
-#define MHD_HALF(f)                            \
-    {                                          \
-        const double h = hdtd * (fr.f - fl.f); \
-        cl.f = ul.f - h;                       \
-        cr.f = ur.f - h;                       \
-    }
-    MHD_HALF(r) MHD_HALF(mn) MHD_HALF(mt) MHD_HALF(mz) MHD_HALF(bn) MHD_HALF(bt) MHD_HALF(bz) MHD_HALF(e)
-    MHD_HALF(psi)
-#undef MHD_HALF
-    const bool pcl = is_physical(to_prim(cl, ph));
-    const bool pcr = is_physical(to_prim(cr, ph));
     // lo = pcl ? cl : ul, hi = pcr ? cr : ur; cl = ul - h, cr = ur - h
     Cons ulb = cons_zero(), urb = cons_zero(), hb = cons_zero();
     if (pcl) { acc(ulb, lob); acc_scaled(hb, -1.0, lob); } else acc(ulb, lob);
@@ -978,6 +960,16 @@ MHD_DI void reconstruct_adj(const Prim& wm, const Prim& wc, const Prim& wp, doub
     MHD_SLOPE_ADJ(r) MHD_SLOPE_ADJ(un) MHD_SLOPE_ADJ(ut) MHD_SLOPE_ADJ(uz) MHD_SLOPE_ADJ(bn) MHD_SLOPE_ADJ(bt)
     MHD_SLOPE_ADJ(bz) MHD_SLOPE_ADJ(p) MHD_SLOPE_ADJ(psi)
 #undef MHD_SLOPE_ADJ
+}
+
+// The same, recomputing the forward values.
+template <int LIMITER>
+MHD_DI void reconstruct_adj(const Prim& wm, const Prim& wc, const Prim& wp, double hdtd, const Phys& ph,
+                            const Cons& lob, const Cons& hib, Prim& wmb, Prim& wcb, Prim& wpb) {
+    Face lo, hi;
+    ReconRecord rec;
+    reconstruct<LIMITER>(wm, wc, wp, hdtd, ph, lo, hi, &rec);
+    reconstruct_adj_rec<LIMITER>(wm, wc, wp, hdtd, ph, rec, lob, hib, wmb, wcb, wpb);
 }
 
 }  // namespace mhd

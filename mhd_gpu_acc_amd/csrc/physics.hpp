@@ -15,6 +15,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 // Measurement knob (never the default): built with -DMHD_FP_CONTRACT_FAST=1 the forward
 // kernels let the compiler fuse multiplies and adds, to time what bitwise parity with
 // the reference's non-contracting flags (Makefile:8,18) costs.  Results then differ
@@ -415,21 +417,35 @@ struct Face {
     Prim w;
 };
 
-template <int LIMITER>
+// What the adjoint of reconstruct() needs of its forward values (adjoint.hpp):
+// the limited slopes, the conserved interface states before the half step and the
+// outcome of the four physical-state guards.  reconstruct() fills it when given one.
+struct RecordNone {};
+struct ReconRecord {
+    double s[9];           // slopes, field order of Prim
+    Cons ul, ur;           // to_cons of the guarded interface primitives
+    bool pl, pr, pcl, pcr; // guards: W_L, W_R, then the half-stepped states
+};
+
+template <int LIMITER, class Record = RecordNone>
 MHD_DI void reconstruct(const Prim& wm, const Prim& wc, const Prim& wp, double hdtd, const Phys& ph, Face& lo,
-                        Face& hi) {
+                        Face& hi, Record* rec = nullptr) {
+    constexpr bool REC = !std::is_same<Record, RecordNone>::value;
     Prim wl, wr;
-#define MHD_SLOPE(f)                                          \
+#define MHD_SLOPE(k, f)                                       \
     {                                                         \
         const double s = limit<LIMITER>(wc.f - wm.f, wp.f - wc.f); \
+        if constexpr (REC) rec->s[k] = s;                     \
         wl.f = wc.f - 0.5 * s;                                \
         wr.f = wc.f + 0.5 * s;                                \
     }
-    MHD_SLOPE(r) MHD_SLOPE(un) MHD_SLOPE(ut) MHD_SLOPE(uz) MHD_SLOPE(bn) MHD_SLOPE(bt) MHD_SLOPE(bz) MHD_SLOPE(p)
-    MHD_SLOPE(psi)
+    MHD_SLOPE(0, r) MHD_SLOPE(1, un) MHD_SLOPE(2, ut) MHD_SLOPE(3, uz) MHD_SLOPE(4, bn) MHD_SLOPE(5, bt)
+    MHD_SLOPE(6, bz) MHD_SLOPE(7, p) MHD_SLOPE(8, psi)
 #undef MHD_SLOPE
-    if (!is_physical(wl)) wl = wc;
-    if (!is_physical(wr)) wr = wc;
+    const bool pl = is_physical(wl);
+    const bool pr = is_physical(wr);
+    if (!pl) wl = wc;
+    if (!pr) wr = wc;
     const Cons ul = to_cons(wl, ph);
     const Cons ur = to_cons(wr, ph);
     const Prim vl = to_prim(ul, ph);
@@ -448,8 +464,18 @@ MHD_DI void reconstruct(const Prim& wm, const Prim& wc, const Prim& wp, double h
 #undef MHD_HALF
     const Prim pcl = to_prim(cl, ph);
     const Prim pcr = to_prim(cr, ph);
-    if (is_physical(pcl)) { lo.u = cl; lo.w = pcl; } else { lo.u = ul; lo.w = vl; }
-    if (is_physical(pcr)) { hi.u = cr; hi.w = pcr; } else { hi.u = ur; hi.w = vr; }
+    const bool gl = is_physical(pcl);
+    const bool gr = is_physical(pcr);
+    if (gl) { lo.u = cl; lo.w = pcl; } else { lo.u = ul; lo.w = vl; }
+    if (gr) { hi.u = cr; hi.w = pcr; } else { hi.u = ur; hi.w = vr; }
+    if constexpr (REC) {
+        rec->ul = ul;
+        rec->ur = ur;
+        rec->pl = pl;
+        rec->pr = pr;
+        rec->pcl = gl;
+        rec->pcr = gr;
+    }
 }
 
 // Conservative update with the non-finite revert (solver_cpu.cpp:386-392).
