@@ -84,7 +84,7 @@ int dt_from_state(mhd_run* r, const double* U, bool reuse, double* dt) {
     return MHD_OK;
 }
 
-int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt, bool want_cfl) {
+int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt, bool want_cfl, bool x_only) {
     mhd::StepArgs a{};
     a.Uold = Uold;
     a.Utmp = Utmp;
@@ -112,7 +112,12 @@ int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, doubl
     a.acc_host = want_cfl ? r->host_acc_dev : nullptr;
     a.stream = r->stream();
     a.ev = nullptr;
-    if (r->profiling) {
+    a.x_only = x_only;
+    if (x_only) {
+        a.acc = nullptr;
+        a.acc_host = nullptr;
+    }
+    if (r->profiling && !x_only) {
         const size_t base = r->events.size();
         for (int k = 0; k < 5; ++k) {
             hipEvent_t ev;
@@ -124,7 +129,7 @@ int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, doubl
     }
     r->host_acc_fresh = false;
     MHD_HIP(mhd::launch_step(a));
-    r->host_acc_fresh = want_cfl && mhd::fused_cfl();
+    r->host_acc_fresh = want_cfl && !x_only && mhd::fused_cfl();
     return MHD_OK;
 }
 

@@ -167,6 +167,7 @@ static hipError_t launch_step_sl(const StepArgs& a) {
                        a.bc[1], a.bc[2], a.bc[3], (const double*)nullptr, (const unsigned long long*)nullptr,
                        (unsigned long long*)nullptr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (a.x_only) return hipSuccess;
     if (a.ev && (e = hipEventRecord(a.ev[2], a.stream)) != hipSuccess) return e;
     // Rows per lane segment: enough column-waves x segments for 2 waves per SIMD
     // (8 per CU), at most 32 rows (the march's 2-row halo is then 6 %) and at least 4.

@@ -30,6 +30,7 @@ struct StepArgs {
                                    // the last ghost fill copies acc into
     hipStream_t stream;
     hipEvent_t* ev;          // null, or 5 events recorded around the 4 launches
+    bool x_only;             // only the x sweep and its ghost fill (Utmp); Unew is not formed
 };
 
 hipError_t launch_step(const StepArgs& a);

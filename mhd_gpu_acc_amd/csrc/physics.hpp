@@ -442,10 +442,15 @@ MHD_DI void reconstruct(const Prim& wm, const Prim& wc, const Prim& wp, double h
     MHD_SLOPE(0, r) MHD_SLOPE(1, un) MHD_SLOPE(2, ut) MHD_SLOPE(3, uz) MHD_SLOPE(4, bn) MHD_SLOPE(5, bt)
     MHD_SLOPE(6, bz) MHD_SLOPE(7, p) MHD_SLOPE(8, psi)
 #undef MHD_SLOPE
-    const bool pl = is_physical(wl);
-    const bool pr = is_physical(wr);
-    if (!pl) wl = wc;
-    if (!pr) wr = wc;
+    if constexpr (REC) {
+        rec->pl = is_physical(wl);
+        rec->pr = is_physical(wr);
+        if (!rec->pl) wl = wc;
+        if (!rec->pr) wr = wc;
+    } else {
+        if (!is_physical(wl)) wl = wc;
+        if (!is_physical(wr)) wr = wc;
+    }
     const Cons ul = to_cons(wl, ph);
     const Cons ur = to_cons(wr, ph);
     const Prim vl = to_prim(ul, ph);
@@ -464,17 +469,16 @@ MHD_DI void reconstruct(const Prim& wm, const Prim& wc, const Prim& wp, double h
 #undef MHD_HALF
     const Prim pcl = to_prim(cl, ph);
     const Prim pcr = to_prim(cr, ph);
-    const bool gl = is_physical(pcl);
-    const bool gr = is_physical(pcr);
-    if (gl) { lo.u = cl; lo.w = pcl; } else { lo.u = ul; lo.w = vl; }
-    if (gr) { hi.u = cr; hi.w = pcr; } else { hi.u = ur; hi.w = vr; }
     if constexpr (REC) {
+        rec->pcl = is_physical(pcl);
+        rec->pcr = is_physical(pcr);
         rec->ul = ul;
         rec->ur = ur;
-        rec->pl = pl;
-        rec->pr = pr;
-        rec->pcl = gl;
-        rec->pcr = gr;
+        if (rec->pcl) { lo.u = cl; lo.w = pcl; } else { lo.u = ul; lo.w = vl; }
+        if (rec->pcr) { hi.u = cr; hi.w = pcr; } else { hi.u = ur; hi.w = vr; }
+    } else {
+        if (is_physical(pcl)) { lo.u = cl; lo.w = pcl; } else { lo.u = ul; lo.w = vl; }
+        if (is_physical(pcr)) { hi.u = cr; hi.w = pcr; } else { hi.u = ur; hi.w = vr; }
     }
 }
 

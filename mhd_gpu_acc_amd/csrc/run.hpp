@@ -44,8 +44,10 @@ int hip_fail(hipError_t e, const char* what);
 int dt_from_state(mhd_run* r, const double* U, bool reuse, double* dt);
 // One advance_second_order from Uold through Utmp into Unew with the run's c_h.
 // want_cfl: the y sweep also reduces Unew's CFL maximum into the run's accumulator
-// (the next compute_dt); a replay with recorded dt / c_h skips it.
-int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt, bool want_cfl = true);
+// (the next compute_dt); a replay with recorded dt / c_h skips it.  x_only: only the x
+// sweep and the ghost fill of Utmp (a replay that needs U* of its last step, not U_next).
+int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt, bool want_cfl = true,
+               bool x_only = false);
 
 }  // namespace mhd
 

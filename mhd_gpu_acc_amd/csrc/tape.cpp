@@ -244,9 +244,13 @@ int mhd_tape_backward(mhd_tape* t, const double* lam_final, double* lam0, int la
         const int s0 = k * t->K, m = std::min(t->K, t->nsteps - s0);
         if (!(kept && k == t->nck - 1)) {
             MHD_HIP(hipMemcpyAsync(t->S[0], t->ck[k], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
+            // The reverse steps read U and U* of each step; the state after the segment
+            // is only needed for the last segment's finiteness check, so elsewhere the
+            // last step stops after its x sweep.
             for (int q = 0; q < m && !rc; ++q) {
                 r->ch = t->chs[s0 + q];
-                rc = mhd::step_grids(r, t->S[q], t->T[q], t->S[q + 1], t->dts[s0 + q], false);
+                const bool x_only = q == m - 1 && k != t->nck - 1;
+                rc = mhd::step_grids(r, t->S[q], t->T[q], t->S[q + 1], t->dts[s0 + q], false, x_only);
             }
         }
         if (rc) break;
