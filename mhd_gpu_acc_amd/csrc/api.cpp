@@ -53,12 +53,30 @@ int check_desc(const mhd_run_desc* d) {
     return MHD_OK;
 }
 
-// Host read of the device CFL maximum (a non-negative double as raw bits). After a
-// step the closing ghost fill has already stored it into host_acc; otherwise copy it.
+// Host read of the device CFL maximum (a non-negative double as raw bits).  After a
+// step the closing ghost fill stores it into host_acc[0] and then the step's number into
+// host_acc[1]: the host polls the number instead of waiting for the stream, which saves
+// the stream synchronisation's latency on every step.  The poll asks the stream every
+// 1024 turns whether it failed or finished without the number arriving.  Otherwise the
+// maximum is copied and the stream synchronised.
 int read_acc(mhd_run* r, double* out) {
-    if (!r->host_acc_fresh)
+    if (r->host_acc_fresh) {
+        for (unsigned spin = 1;; ++spin) {
+            if (__atomic_load_n(r->host_acc + 1, __ATOMIC_ACQUIRE) == r->seq) break;
+            if ((spin & 1023u) == 0) {
+                const hipError_t q = hipStreamQuery(r->stream());
+                if (q == hipSuccess) {
+                    if (__atomic_load_n(r->host_acc + 1, __ATOMIC_ACQUIRE) == r->seq) break;
+                    return fail(MHD_ERR_HIP, "compute_dt: the step finished without handing over its CFL maximum");
+                }
+                if (q != hipErrorNotReady) return hip_fail(q, "hipStreamQuery");
+            }
+            __builtin_ia32_pause();
+        }
+    } else {
         MHD_HIP(hipMemcpyAsync(r->host_acc, r->acc, sizeof(unsigned long long), hipMemcpyDeviceToHost, r->stream()));
-    MHD_HIP(hipStreamSynchronize(r->stream()));
+        MHD_HIP(hipStreamSynchronize(r->stream()));
+    }
     unsigned long long bits = __atomic_load_n(r->host_acc, __ATOMIC_ACQUIRE);
     std::memcpy(out, &bits, sizeof bits);
     return MHD_OK;
@@ -110,6 +128,7 @@ int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, doubl
     a.frame = r->frame;
     a.acc = want_cfl ? r->acc : nullptr;
     a.acc_host = want_cfl ? r->host_acc_dev : nullptr;
+    a.seq = (want_cfl && !x_only) ? ++r->seq : 0;
     a.stream = r->stream();
     a.ev = nullptr;
     a.x_only = x_only;
@@ -193,12 +212,13 @@ int mhd_run_create(const mhd_run_desc* desc, mhd_run** out) {
     hipError_t e = hipMalloc(&r->frame, 4 * (size_t)(desc->nx + desc->ny) * sizeof(double));
     if (e != hipSuccess) return cleanup(e, "hipMalloc(frame)");
     if ((e = hipMalloc(&r->acc, sizeof(unsigned long long))) != hipSuccess) return cleanup(e, "hipMalloc(acc)");
-    if ((e = hipHostMalloc(&r->host_acc, sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent)) !=
+    if ((e = hipHostMalloc(&r->host_acc, 2 * sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent)) !=
         hipSuccess)
         return cleanup(e, "hipHostMalloc");
     if ((e = hipHostGetDevicePointer((void**)&r->host_acc_dev, r->host_acc, 0)) != hipSuccess)
         return cleanup(e, "hipHostGetDevicePointer");
-    *r->host_acc = 0;
+    r->host_acc[0] = 0;
+    r->host_acc[1] = 0;
     if ((e = hipEventCreateWithFlags(&r->order, hipEventDisableTiming)) != hipSuccess)
         return cleanup(e, "hipEventCreate");
     if ((e = hipStreamSynchronize(r->stream())) != hipSuccess) return cleanup(e, "hipStreamSynchronize");

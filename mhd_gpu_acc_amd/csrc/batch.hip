@@ -48,7 +48,7 @@ struct BatchStep {
     Phys ph;
     double dtdx, hdtdx, dtdy, hdtdy, damp;
     int cur;        // 0 or 2
-    int pad;
+    unsigned seq;   // the step's number (read from run 0's record), handed to the host last
 };
 
 MHD_DI Geom geom_of(const BatchRun& R) { return Geom{R.nx, R.ny, R.nx + 2 * NG, R.ny + 2 * NG, R.ps}; }
@@ -147,9 +147,17 @@ __global__ __launch_bounds__(256) void batch_bc_kernel(const BatchRun* __restric
                                                        const int2* __restrict__ tiles, int phase,
                                                        const unsigned long long* __restrict__ acc,
                                                        unsigned long long* acc_host, int nruns) {
-    if (acc_host && blockIdx.x == 0)
+    if (acc_host && blockIdx.x == 0) {
+        // every run's CFL maximum, then the step's number after all of them (a polling host
+        // reads the maxima once the number has arrived)
         for (int r = threadIdx.x; r < nruns; r += blockDim.x)
             __hip_atomic_store(acc_host + r, acc[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __threadfence_system();
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(acc_host + nruns, (unsigned long long)steps[0].seq, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+    }
     const int2 tl = tiles[blockIdx.x];
     const BatchRun& R = runs[tl.x];
     const BatchStep& P = steps[tl.x];
@@ -196,7 +204,9 @@ struct mhd_batch {
     BatchStep* d_steps = nullptr;
     BatchStep* h_steps = nullptr;        // pinned
     unsigned long long* d_acc = nullptr;
-    unsigned long long* h_acc = nullptr; // pinned, coherent; written by the step's last ghost fill
+    unsigned long long* h_acc = nullptr; // pinned, coherent; written by the step's last ghost fill:
+                                         // n maxima, then the step's number
+    unsigned seq = 0;                    // number of the last step launched
     unsigned long long* h_acc_dev = nullptr;  // its device address
     int cur = 0;
     bool acc_valid = false;
@@ -355,9 +365,29 @@ int plan_step(mhd_batch* b, double* dts, int* bad_run) {
         P.hdtdy = 0.5 * P.dtdy;
         P.damp = std::exp(-dt * mx / b->d[r].cr);
         P.cur = b->cur;
+        P.seq = b->seq + 1;
         if (dts) dts[r] = dt;
     }
     return MHD_OK;
+}
+
+// Waits until the step numbered b->seq has handed its maxima over (the closing ghost
+// fill stores the number after them), polling pinned memory instead of synchronising
+// the stream; every 1024 turns it asks the stream whether it failed or finished.
+int wait_handover(mhd_batch* b) {
+    for (unsigned spin = 1;; ++spin) {
+        if (__atomic_load_n(b->h_acc + b->n, __ATOMIC_ACQUIRE) == (unsigned long long)b->seq) return MHD_OK;
+        if ((spin & 1023u) == 0) {
+            const hipError_t q = hipStreamQuery(b->stream);
+            if (q == hipSuccess) {
+                if (__atomic_load_n(b->h_acc + b->n, __ATOMIC_ACQUIRE) == (unsigned long long)b->seq) return MHD_OK;
+                return set_error(MHD_ERR_HIP, "batch step finished without handing over its CFL maxima");
+            }
+            if (q != hipErrorNotReady)
+                return set_error(MHD_ERR_HIP, (std::string("hipStreamQuery: ") + hipGetErrorString(q)).c_str());
+        }
+        __builtin_ia32_pause();
+    }
 }
 
 int check_run(const mhd_batch* b, int r) {
@@ -431,11 +461,12 @@ int mhd_batch_create(const mhd_run_desc* descs, int n, mhd_batch** out) {
         hipMalloc(&b->d_acc, n * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&b->d_queue, sizeof(unsigned int)) != hipSuccess ||
         hipHostMalloc(&b->h_steps, n * sizeof(BatchStep), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(&b->h_acc, n * sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent) !=
+        hipHostMalloc(&b->h_acc, (n + 1) * sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent) !=
             hipSuccess ||
         hipHostGetDevicePointer((void**)&b->h_acc_dev, b->h_acc, 0) != hipSuccess)
         return bail(set_error(MHD_ERR_HIP, "batch allocation"));
     std::memset(b->h_steps, 0, n * sizeof(BatchStep));
+    std::memset(b->h_acc, 0, (n + 1) * sizeof(unsigned long long));
     if (int rc = build_tiles(b)) return bail(rc);
     if (hipStreamSynchronize(b->stream) != hipSuccess) return bail(set_error(MHD_ERR_HIP, "sync"));
     *out = b;
@@ -584,7 +615,8 @@ int mhd_batch_step(mhd_batch* b, int nsteps, double* dts, int* done) {
             return set_error(MHD_ERR_NO_SIGNAL, msg);
         }
         B_HIP(hipGraphLaunch(b->exec, b->stream));
-        B_HIP(hipStreamSynchronize(b->stream));
+        b->seq += 1;
+        if (int rc = wait_handover(b)) return rc;
         b->cur = 2 - b->cur;
         if (done) *done = s + 1;
     }
@@ -665,6 +697,9 @@ done:
         if (p) (void)hipFree(p);
     b->ch = ch0;
     std::memcpy(b->h_acc, acc0.data(), b->n * sizeof(unsigned long long));
+    // the replays handed over the number of the next step: take it back, so that the next
+    // step's poll waits for that step's own hand-over (the stream is idle here)
+    __atomic_store_n(b->h_acc + b->n, (unsigned long long)b->seq, __ATOMIC_RELEASE);
     b->acc_valid = true;
     if (rc != MHD_OK) {
         std::memcpy(best, best0, sizeof best);

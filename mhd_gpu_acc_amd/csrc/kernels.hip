@@ -85,13 +85,17 @@ __global__ __launch_bounds__(256) void bc_kernel(double* __restrict__ U, const d
                                                  int bc0, int bc1, int bc2, int bc3,
                                                  const double* __restrict__ frame,
                                                  const unsigned long long* __restrict__ acc,
-                                                 unsigned long long* acc_host) {
+                                                 unsigned long long* acc_host, unsigned long long seq) {
     const int bc[4] = {bc0, bc1, bc2, bc3};
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < ghost_count(g)) ghost_fill(U, prev, g, bc, frame, t);
-    // Hand the previous launch's CFL maximum to the host: one system-scope store into
-    // pinned memory replaces a device-to-host copy after the step.
-    if (acc_host && t == 0) __hip_atomic_store(acc_host, *acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    // Hand the previous launch's CFL maximum to the host: a system-scope store into pinned
+    // memory replaces a device-to-host copy after the step, and the step's number, stored
+    // after it with release order, tells a polling host that it has arrived.
+    if (acc_host && t == 0) {
+        __hip_atomic_store(acc_host, *acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(acc_host + 1, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -165,7 +169,7 @@ static hipError_t launch_step_sl(const StepArgs& a) {
     const int nbc = 2 * NG * g.ny + 2 * NG * g.tnx;
     hipLaunchKernelGGL(bc_kernel, dim3((nbc + 255) / 256), dim3(256), 0, a.stream, a.Utmp, a.Uold, g, a.bc[0],
                        a.bc[1], a.bc[2], a.bc[3], (const double*)nullptr, (const unsigned long long*)nullptr,
-                       (unsigned long long*)nullptr);
+                       (unsigned long long*)nullptr, 0ull);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.x_only) return hipSuccess;
     if (a.ev && (e = hipEventRecord(a.ev[2], a.stream)) != hipSuccess) return e;
@@ -190,7 +194,7 @@ static hipError_t launch_step_sl(const StepArgs& a) {
     const bool to_host = MHD_FUSED_CFL && a.acc && a.acc_host;
     hipLaunchKernelGGL(bc_kernel, dim3((nbc + 255) / 256), dim3(256), 0, a.stream, a.Unew, a.Utmp, g, a.bc[0],
                        a.bc[1], a.bc[2], a.bc[3], (const double*)a.frame, (const unsigned long long*)a.acc,
-                       to_host ? a.acc_host : (unsigned long long*)nullptr);
+                       to_host ? a.acc_host : (unsigned long long*)nullptr, a.seq);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.ev && (e = hipEventRecord(a.ev[4], a.stream)) != hipSuccess) return e;
     return hipSuccess;
@@ -233,7 +237,7 @@ hipError_t launch_bc(double* U, const double* prev, int nx, int ny, long long ps
     const Geom g{nx, ny, nx + 2 * NG, ny + 2 * NG, ps};
     const int nbc = 2 * NG * g.ny + 2 * NG * g.tnx;
     hipLaunchKernelGGL(bc_kernel, dim3((nbc + 255) / 256), dim3(256), 0, s, U, prev, g, bc[0], bc[1], bc[2], bc[3],
-                       (const double*)nullptr, (const unsigned long long*)nullptr, (unsigned long long*)nullptr);
+                       (const double*)nullptr, (const unsigned long long*)nullptr, (unsigned long long*)nullptr, 0ull);
     return hipGetLastError();
 }
 

@@ -27,7 +27,8 @@ struct StepArgs {
     double* frame;           // undamped psi of the interior frame (2*2*(nx+ny) doubles)
     unsigned long long* acc; // max signal speed accumulator (bit pattern of a double >= 0)
     unsigned long long* acc_host;  // null, or the device address of pinned host memory that
-                                   // the last ghost fill copies acc into
+                                   // the last ghost fill copies acc into ([0]), then seq ([1])
+    unsigned long long seq;        // this step's number, stored after the maximum
     hipStream_t stream;
     hipEvent_t* ev;          // null, or 5 events recorded around the 4 launches
     bool x_only;             // only the x sweep and its ghost fill (Utmp); Unew is not formed

@@ -17,9 +17,10 @@ struct mhd_run {
     double* U[3] = {nullptr, nullptr, nullptr};  // current, intermediate (U*), next
     double* frame = nullptr;
     unsigned long long* acc = nullptr;
-    unsigned long long* host_acc = nullptr;      // pinned, coherent
+    unsigned long long* host_acc = nullptr;      // pinned, coherent: [0] CFL maximum, [1] step number
     unsigned long long* host_acc_dev = nullptr;  // its device address
     bool host_acc_fresh = false;  // the last step's ghost fill stores acc into host_acc
+    unsigned long long seq = 0;   // number of the last step that hands its maximum over
     double ch = 0.0;                         // c_h, starts at 0 (physics.hpp:41)
     bool acc_valid = false;                  // acc holds the CFL max of U[0] with the current ch
     int seg = 0;
