@@ -55,7 +55,10 @@ MHD_DI Geom geom_of(const BatchRun& R) { return Geom{R.nx, R.ny, R.nx + 2 * NG, 
 // Same operations as the host builds for a plain launch (kernels.hip launch_step_sl).
 inline Phys phys_of(double gamma, double ch) { return Phys{gamma, gamma - 1.0, ch, ch * ch, 0.5 * ch, 0.5 / ch}; }
 
-template <int SOLVER, int LIMITER, int WG>
+// BUF: every run's grid is addressed through one buffer resource (all grids of the batch
+// are at most BUF_MAX_BYTES), as the plain sweeps; else through global-address-space
+// plane views.
+template <int SOLVER, int LIMITER, int WG, bool BUF>
 __global__ __launch_bounds__(WG) void batch_xsweep_kernel(const BatchRun* __restrict__ runs,
                                                           const BatchStep* __restrict__ steps,
                                                           const int2* __restrict__ tiles, int ntiles,
@@ -75,7 +78,8 @@ __global__ __launch_bounds__(WG) void batch_xsweep_kernel(const BatchRun* __rest
         const BatchRun& R = runs[tl.x];
         const BatchStep& P = steps[tl.x];
         const Geom g = geom_of(R);
-        const Planes<true, GlobalT<const double>> src(as_global<const double>(R.U[P.cur]), g.ps);
+        using Src = std::conditional_t<BUF, BufPlanes<true>, Planes<true, GlobalT<const double>>>;
+        const Src src = plane_view<BUF, Src>(static_cast<const double*>(R.U[P.cur]), g.ps);
         const int k = NG * g.tnx - 2 + tl.y * (WG - 4) + t;
         return src.load(min(k, g.tnx * g.tny - 1));
     };
@@ -93,7 +97,8 @@ __global__ __launch_bounds__(WG) void batch_xsweep_kernel(const BatchRun* __rest
         const BatchStep& P = steps[tl.x];
         const Geom g = geom_of(R);
         const Phys ph = P.ph;
-        const Planes<true, GlobalT<double>> dst(as_global(R.U[1]), g.ps);
+        using Dst = std::conditional_t<BUF, BufPlanes<true>, Planes<true, GlobalT<double>>>;
+        const Dst dst = plane_view<BUF, Dst>(R.U[1], g.ps);
         const int k = NG * g.tnx - 2 + tl.y * (WG - 4) + t;
         xsweep_segment<SOLVER, LIMITER, WG>(u, k, t, dst, g, ph, P.dtdx, P.hdtdx, L);
         if (qn >= ntiles) break;
@@ -109,7 +114,7 @@ __global__ __launch_bounds__(WG) void batch_xsweep_kernel(const BatchRun* __rest
 // the tail of the launch.
 // Two waves per SIMD: HLLD otherwise takes 256 VGPRs plus a few AGPRs, which
 // halves the occupancy.
-template <int SOLVER, int LIMITER>
+template <int SOLVER, int LIMITER, bool BUF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void batch_ysweep_kernel(const BatchRun* __restrict__ runs,
                                                            const BatchStep* __restrict__ steps,
                                                            const int4* __restrict__ tiles, int ntiles,
@@ -132,7 +137,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
         const int i = NG + tl.y * 64 + lane;
         const int j0 = NG + tl.z;
         const int j1 = j0 + tl.w;
-        const double cand = ysweep_column<SOLVER, LIMITER>(R.U[1], R.U[2 - P.cur], g, P.ph, P.dtdy, P.hdtdy,
+        const double cand = ysweep_column<SOLVER, LIMITER, BUF>(R.U[1], R.U[2 - P.cur], g, P.ph, P.dtdy, P.hdtdy,
                                                            P.damp, R.frame, true, i, i < NG + g.nx, j0, j1, yr);
         wave_max_to(acc + tl.x, cand);
     }
@@ -210,12 +215,14 @@ struct mhd_batch {
     unsigned long long* h_acc_dev = nullptr;  // its device address
     int cur = 0;
     bool acc_valid = false;
+    bool buf = true;  // every grid within BUF_MAX_BYTES: buffer-resource addressing
     // launch shape (autotuned)
     int xwg = 256, xgrid = 8, yseg = 64, ygrid = 8;  // yseg: tall tile height; tails use yseg/4
     unsigned int* d_queue = nullptr;
     int2* d_xt = nullptr;  int nxt = 0;
     int4* d_yt = nullptr;  int nyt = 0;
-    int2* d_bt = nullptr;  int nbt = 0;
+    int2* d_bt = nullptr;  int nbt = 0;   // ghost cells of the next state (phase 1)
+    int2* d_bt0 = nullptr; int nbt0 = 0;  // ghost cells of U* the y sweep reads (phase 0)
     int2* d_ct = nullptr;  int nct = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -240,7 +247,7 @@ template <class T> int upload(T** dptr, const std::vector<T>& v) {
 }
 
 int build_tiles(mhd_batch* b) {
-    std::vector<int2> xt, bt, ct;
+    std::vector<int2> xt, bt, bt0, ct;
     std::vector<int4> yt;
     for (int r = 0; r < b->n; ++r) {
         const int nx = b->d[r].nx, ny = b->d[r].ny, tnx = nx + 4;
@@ -259,34 +266,52 @@ int build_tiles(mhd_batch* b) {
         }
         const int ng = 2 * 2 * ny + 2 * 2 * tnx;
         for (int c = 0; c < (ng + 255) / 256; ++c) bt.push_back(make_int2(r, c));
+        // U* is read only by the y sweep (its ghost rows) and, through a Dirichlet side, by
+        // the next state's fill; without such a side its ghost columns (the first 2 NG ny
+        // ghost-cell numbers) are never read, so its fill starts at the chunk holding the
+        // first ghost-row cell.
+        const bool dirichlet = b->d[r].bc[0] == 3 || b->d[r].bc[1] == 3 || b->d[r].bc[2] == 3 || b->d[r].bc[3] == 3;
+        for (int c = dirichlet ? 0 : (2 * 2 * ny) / 256; c < (ng + 255) / 256; ++c) bt0.push_back(make_int2(r, c));
         for (int c = 0; c < (nx * ny + 255) / 256; ++c) ct.push_back(make_int2(r, c));
     }
     std::stable_sort(yt.begin(), yt.end(), [](const int4& a, const int4& c) { return a.w > c.w; });
     b->nxt = (int)xt.size();
     b->nyt = (int)yt.size();
     b->nbt = (int)bt.size();
+    b->nbt0 = (int)bt0.size();
     b->nct = (int)ct.size();
     if (int rc = upload(&b->d_xt, xt)) return rc;
     if (int rc = upload(&b->d_yt, yt)) return rc;
     if (int rc = upload(&b->d_bt, bt)) return rc;
+    if (int rc = upload(&b->d_bt0, bt0)) return rc;
     return upload(&b->d_ct, ct);
 }
 
-template <int S, int L, int WG> void launch_bx(mhd_batch* b) {
+template <int S, int L, int WG, bool BUF> void launch_bx(mhd_batch* b) {
     const int nb = std::min(b->nxt, b->xgrid * b->cus);
-    hipLaunchKernelGGL((batch_xsweep_kernel<S, L, WG>), dim3(std::max(nb, 1)), dim3(WG), 0, b->stream, b->d_runs,
-                       b->d_steps, b->d_xt, b->nxt, b->d_acc, b->n, b->d_queue);
+    hipLaunchKernelGGL((batch_xsweep_kernel<S, L, WG, BUF>), dim3(std::max(nb, 1)), dim3(WG), 0, b->stream,
+                       b->d_runs, b->d_steps, b->d_xt, b->nxt, b->d_acc, b->n, b->d_queue);
+}
+
+template <int S, int L, bool BUF> void launch_by(mhd_batch* b) {
+    const int wpb = 4;
+    const int nb = std::min((b->nyt + wpb - 1) / wpb, b->ygrid * b->cus);
+    hipLaunchKernelGGL((batch_ysweep_kernel<S, L, BUF>), dim3(std::max(nb, 1)), dim3(64 * wpb), 0, b->stream,
+                       b->d_runs, b->d_steps, b->d_yt, b->nyt, b->d_queue, b->d_acc);
 }
 
 template <int S, int L> hipError_t launch_batch_step_sl(mhd_batch* b) {
-    if (b->xwg == 128) launch_bx<S, L, 128>(b);
-    else launch_bx<S, L, 256>(b);
-    hipLaunchKernelGGL(batch_bc_kernel, dim3(b->nbt), dim3(256), 0, b->stream, b->d_runs, b->d_steps, b->d_bt, 0,
+    if (b->buf) {
+        if (b->xwg == 128) launch_bx<S, L, 128, true>(b);
+        else launch_bx<S, L, 256, true>(b);
+    } else {
+        if (b->xwg == 128) launch_bx<S, L, 128, false>(b);
+        else launch_bx<S, L, 256, false>(b);
+    }
+    hipLaunchKernelGGL(batch_bc_kernel, dim3(b->nbt0), dim3(256), 0, b->stream, b->d_runs, b->d_steps, b->d_bt0, 0,
                        (const unsigned long long*)nullptr, (unsigned long long*)nullptr, 0);
-    const int wpb = 4;
-    const int nb = std::min((b->nyt + wpb - 1) / wpb, b->ygrid * b->cus);
-    hipLaunchKernelGGL((batch_ysweep_kernel<S, L>), dim3(std::max(nb, 1)), dim3(64 * wpb), 0, b->stream, b->d_runs,
-                       b->d_steps, b->d_yt, b->nyt, b->d_queue, b->d_acc);
+    if (b->buf) launch_by<S, L, true>(b);
+    else launch_by<S, L, false>(b);
     hipLaunchKernelGGL(batch_bc_kernel, dim3(b->nbt), dim3(256), 0, b->stream, b->d_runs, b->d_steps, b->d_bt, 1,
                        (const unsigned long long*)b->d_acc, b->h_acc_dev, b->n);
     return hipGetLastError();
@@ -437,6 +462,8 @@ int mhd_batch_create(const mhd_run_desc* descs, int n, mhd_batch** out) {
         b->own_stream = true;
     }
     b->runs.resize(n);
+    const char* addr = std::getenv("MHD_ADDR");  // "flat": the global-address-space views
+    b->buf = !(addr && std::strcmp(addr, "flat") == 0);
     for (int r = 0; r < n; ++r) {
         const mhd_run_desc& d = descs[r];
         b->dx.push_back((d.x_max - d.x_min) / static_cast<double>(d.nx));
@@ -445,6 +472,7 @@ int mhd_batch_create(const mhd_run_desc* descs, int n, mhd_batch** out) {
         R.nx = d.nx;
         R.ny = d.ny;
         R.ps = (long long)(d.nx + 4) * (d.ny + 4);
+        if (9ull * (unsigned long long)R.ps * 8 > BUF_MAX_BYTES) b->buf = false;
         R.gamma = d.gamma;
         std::memcpy(R.bc, d.bc, sizeof R.bc);
         for (int k = 0; k < 3; ++k) {
@@ -483,7 +511,7 @@ int mhd_batch_destroy(mhd_batch* b) {
         if (R.frame) (void)hipFree(R.frame);
     }
     for (void* p : {(void*)b->d_runs, (void*)b->d_steps, (void*)b->d_acc, (void*)b->d_queue, (void*)b->d_xt, (void*)b->d_yt,
-                    (void*)b->d_bt, (void*)b->d_ct})
+                    (void*)b->d_bt, (void*)b->d_bt0, (void*)b->d_ct})
         if (p) (void)hipFree(p);
     if (b->h_steps) (void)hipHostFree(b->h_steps);
     if (b->h_acc) (void)hipHostFree(b->h_acc);
@@ -647,7 +675,7 @@ int mhd_batch_tune(mhd_batch* b, int reps, char* report, size_t report_len) {
     hipEvent_t e0, e1;
     B_HIP(hipEventCreate(&e0));
     B_HIP(hipEventCreate(&e1));
-    const int xwgs[] = {128, 256}, xgrids[] = {8, 16}, ysegs[] = {16, 32, 64}, ygrids[] = {2, 4};
+    const int xwgs[] = {128, 256}, xgrids[] = {8, 16}, ysegs[] = {16, 32, 64, 128}, ygrids[] = {2, 4};
     int rc = MHD_OK;
     for (int xw : xwgs)
         for (int xg : xgrids)

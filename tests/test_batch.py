@@ -115,6 +115,14 @@ def test_batch_equals_plain_runs(solver, limiter):
 
 
 @pytest.mark.gpu
+def test_batch_flat_addressing_equals_plain_runs(monkeypatch):
+    """The batched sweeps' global-address-space form (the one batches with a grid above
+    4 GiB take), forced, against the plain runs."""
+    monkeypatch.setenv("MHD_ADDR", "flat")
+    test_batch_equals_plain_runs(2, 1)
+
+
+@pytest.mark.gpu
 def test_batch_tuning_keeps_results():
     cfgs, states = _mixed_runs()
     say("[gpu] batch tune")
