@@ -87,23 +87,45 @@ MHD_DI Cons gather_adj(const Lam& lam, const Geom& g, const int* bc, int i, int 
     v.psi = v.psi * psi_scale;
     const int li = i - NG, lj = j - NG;
     if (li >= NG && li < g.nx - NG && lj >= NG && lj < g.ny - NG) return v;
-    // Candidates: ghost columns x {j} and ghost rows, {i} x ghost rows, and corners.
-    const int gx[5] = {i, 0, 1, NG + g.nx, NG + g.nx + 1};
-    const int gy[5] = {j, 0, 1, NG + g.ny, NG + g.ny + 1};
+    // Candidates (gx(a), gy(b)), a, b in 0..4: ghost columns x {j} and ghost rows,
+    // {i} x ghost rows, and corners, where gx = {i, 0, 1, nx+2, nx+3} and gy likewise.
+    // ghost_source maps the two axes independently, so a candidate's source is (i, j)
+    // itself exactly when its column maps to i and its row to j, neither through a
+    // Dirichlet side, with the flips of the two maps.  Ten axis maps give the set of
+    // matching candidates as a 25-bit mask (bit 5a + b); its bits are taken in
+    // increasing order, the order of a loop over a then b.
+    unsigned mx = 0, my = 0, fx = 0, fy = 0;
+#pragma unroll
+    for (int a = 0; a < 5; ++a) {
+        const int cx = a == 0 ? i : (a < 3 ? a - 1 : g.nx + a - 1);
+        const int cy = a == 0 ? j : (a < 3 ? a - 1 : g.ny + a - 1);
+        int s;
+        bool f;
+        if (map_axis(cx, g.nx, bc[0], bc[1], s, f) && s == i) mx |= 1u << a;
+        if (f) fx |= 1u << a;
+        if (map_axis(cy, g.ny, bc[2], bc[3], s, f) && s == j) my |= 1u << a;
+        if (f) fy |= 1u << a;
+    }
+    unsigned m = 0;
+#pragma unroll
     for (int a = 0; a < 5; ++a)
-        for (int b = 0; b < 5; ++b) {
-            if (a == 0 && b == 0) continue;
-            const int ci = gx[a], cj = gy[b];
-            const Src s = ghost_source(ci, cj, g, bc);
-            if (s.prev || s.i != i || s.j != j) continue;
-            Cons w = lam(ci, cj);
-            // reference-frame flips: x negates rho*u and Bx, y negates rho*v and By
-            const bool fn = XDIR ? s.flip_x : s.flip_y;  // normal slots
-            const bool ft = XDIR ? s.flip_y : s.flip_x;  // tangential slots
-            if (fn) { w.mn = -w.mn; w.bn = -w.bn; }
-            if (ft) { w.mt = -w.mt; w.bt = -w.bt; }
-            acc(v, w);
-        }
+        if (mx & (1u << a)) m |= my << (5 * a);
+    m &= ~1u;  // (i, j) itself
+    while (m) {
+        const int k = __builtin_ctz(m);
+        m &= m - 1;
+        const int a = k / 5, b = k % 5;
+        const int ci = a == 0 ? i : (a < 3 ? a - 1 : g.nx + a - 1);
+        const int cj = b == 0 ? j : (b < 3 ? b - 1 : g.ny + b - 1);
+        Cons w = lam(ci, cj);
+        // reference-frame flips: x negates rho*u and Bx, y negates rho*v and By
+        const bool flx = (fx >> a) & 1u, fly = (fy >> b) & 1u;
+        const bool fn = XDIR ? flx : fly;  // normal slots
+        const bool ft = XDIR ? fly : flx;  // tangential slots
+        if (fn) { w.mn = -w.mn; w.bn = -w.bn; }
+        if (ft) { w.mt = -w.mt; w.bt = -w.bt; }
+        acc(v, w);
+    }
     return v;
 }
 

@@ -102,7 +102,8 @@ int dt_from_state(mhd_run* r, const double* U, bool reuse, double* dt) {
     return MHD_OK;
 }
 
-int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt, bool want_cfl, bool x_only) {
+int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt, bool want_cfl, bool x_only,
+               hipStream_t stream) {
     mhd::StepArgs a{};
     a.Uold = Uold;
     a.Utmp = Utmp;
@@ -129,14 +130,14 @@ int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, doubl
     a.acc = want_cfl ? r->acc : nullptr;
     a.acc_host = want_cfl ? r->host_acc_dev : nullptr;
     a.seq = (want_cfl && !x_only) ? ++r->seq : 0;
-    a.stream = r->stream();
+    a.stream = stream ? stream : r->stream();
     a.ev = nullptr;
     a.x_only = x_only;
     if (x_only) {
         a.acc = nullptr;
         a.acc_host = nullptr;
     }
-    if (r->profiling && !x_only) {
+    if (r->profiling && !x_only && !stream) {
         const size_t base = r->events.size();
         for (int k = 0; k < 5; ++k) {
             hipEvent_t ev;

@@ -47,8 +47,11 @@ int dt_from_state(mhd_run* r, const double* U, bool reuse, double* dt);
 // want_cfl: the y sweep also reduces Unew's CFL maximum into the run's accumulator
 // (the next compute_dt); a replay with recorded dt / c_h skips it.  x_only: only the x
 // sweep and the ghost fill of Utmp (a replay that needs U* of its last step, not U_next).
+// stream: where to launch (null: the run's stream; a replay on another stream records no
+// profiling events).  The launches use the run's frame buffer, so two steps of one run
+// must not be in flight on different streams at once.
 int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt, bool want_cfl = true,
-               bool x_only = false);
+               bool x_only = false, hipStream_t stream = nullptr);
 
 }  // namespace mhd
 

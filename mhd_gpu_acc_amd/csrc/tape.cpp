@@ -9,7 +9,13 @@
 // keeping every step's input U and intermediate U* -- except the last segment on
 // the first backward after a forward, which the forward left there; then the
 // reverse steps of adjoint.hip run from the segment's last step to its first.
-// Device memory: ceil(nsteps/K) checkpoints + (2K+1) segment grids + 2 adjoint grids.
+// With two or more segments there are two sets of segment buffers: a second stream
+// recomputes the next segment into one set while the run's stream reverses the
+// current one from the other, so the recompute fills what the reverse sweeps leave
+// idle (their tails, the launch gaps).  Events order the two streams; results are
+// the same bits either way.
+// Device memory: ceil(nsteps/K) checkpoints + (2K+1) segment grids per set + 2
+// adjoint grids.  When the second set does not fit, the tape runs with one set.
 //
 // The adjoint treats the whole update as differentiable: the reference's revert of
 // a cell whose update is non-finite (solver_cpu.cpp:386-392) cannot occur on a
@@ -19,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -32,10 +39,15 @@ struct mhd_tape {
     int nsteps = 0, K = 0, nck = 0;
     std::vector<double> dts, chs;
     bool recorded = false;
-    bool last_kept = false;    // S/T hold the last segment as the forward stepped it
+    bool last_kept = false;    // set 0 holds the last segment as the forward stepped it
     std::vector<double*> ck;   // checkpoints
-    std::vector<double*> S;    // segment states, K+1
-    std::vector<double*> T;    // segment intermediates, K
+    struct Set {
+        std::vector<double*> S;  // segment states, K+1
+        std::vector<double*> T;  // segment intermediates, K
+    } set[2];                    // set[1] stays empty when the tape runs with one set
+    bool two = false;            // two sets and a recompute stream
+    hipStream_t side = nullptr;  // recompute stream (default priority)
+    hipEvent_t ev_join = nullptr, ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
     double* lam[2] = {nullptr, nullptr};
     unsigned int* count = nullptr;
     unsigned int* host_count = nullptr;  // pinned
@@ -137,10 +149,16 @@ extern "C" {
 int mhd_tape_destroy(mhd_tape* t) {
     if (!t) return MHD_OK;
     if (t->run) (void)hipStreamSynchronize(t->run->stream());
+    if (t->side) (void)hipStreamSynchronize(t->side);
     for (double* p : t->ck) if (p) (void)hipFree(p);
-    for (double* p : t->S) if (p) (void)hipFree(p);
-    for (double* p : t->T) if (p) (void)hipFree(p);
+    for (auto& st : t->set) {
+        for (double* p : st.S) if (p) (void)hipFree(p);
+        for (double* p : st.T) if (p) (void)hipFree(p);
+    }
     for (double* p : t->lam) if (p) (void)hipFree(p);
+    for (hipEvent_t e : {t->ev_join, t->ev_ready[0], t->ev_ready[1], t->ev_free[0], t->ev_free[1]})
+        if (e) (void)hipEventDestroy(e);
+    if (t->side) (void)hipStreamDestroy(t->side);
     if (t->count) (void)hipFree(t->count);
     if (t->host_count) (void)hipHostFree(t->host_count);
     delete t;
@@ -159,13 +177,43 @@ int mhd_tape_create(mhd_run* r, int nsteps, int K, mhd_tape** out) {
     t->K = std::min(K, nsteps);
     t->nck = (nsteps + t->K - 1) / t->K;
     t->ck.assign(t->nck, nullptr);
-    t->S.assign(t->K + 1, nullptr);
-    t->T.assign(t->K, nullptr);
+    t->set[0].S.assign(t->K + 1, nullptr);
+    t->set[0].T.assign(t->K, nullptr);
     int rc = MHD_OK;
     for (auto& p : t->ck) if (!rc) rc = alloc_grid(r, &p);
-    for (auto& p : t->S) if (!rc) rc = alloc_grid(r, &p);
-    for (auto& p : t->T) if (!rc) rc = alloc_grid(r, &p);
+    for (auto& p : t->set[0].S) if (!rc) rc = alloc_grid(r, &p);
+    for (auto& p : t->set[0].T) if (!rc) rc = alloc_grid(r, &p);
     for (auto& p : t->lam) if (!rc) rc = alloc_grid(r, &p);
+    if (!rc) {
+        hipError_t e = hipEventCreateWithFlags(&t->ev_join, hipEventDisableTiming);
+        for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+            e = hipEventCreateWithFlags(&t->ev_ready[b], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&t->ev_free[b], hipEventDisableTiming);
+        }
+        if (e != hipSuccess) rc = mhd::hip_fail(e, "tape events");
+    }
+    // MHD_TAPE_SETS=1 (a measurement knob): one set, no recompute stream.
+    const char* sets = std::getenv("MHD_TAPE_SETS");
+    if (!rc && t->nck > 1 && !(sets && std::strcmp(sets, "1") == 0)) {
+        // The second set and its stream are an optimisation: without the memory the tape
+        // runs with one set.
+        auto& st = t->set[1];
+        st.S.assign(t->K + 1, nullptr);
+        st.T.assign(t->K, nullptr);
+        bool ok = hipStreamCreateWithFlags(&t->side, hipStreamNonBlocking) == hipSuccess;
+        for (auto& p : st.S) ok = ok && hipMalloc(&p, grid_bytes(r)) == hipSuccess;
+        for (auto& p : st.T) ok = ok && hipMalloc(&p, grid_bytes(r)) == hipSuccess;
+        if (!ok) {
+            (void)hipGetLastError();
+            for (double* p : st.S) if (p) (void)hipFree(p);
+            for (double* p : st.T) if (p) (void)hipFree(p);
+            st.S.clear();
+            st.T.clear();
+            if (t->side) (void)hipStreamDestroy(t->side);
+            t->side = nullptr;
+        }
+        t->two = ok;
+    }
     if (!rc) {
         hipError_t e = hipMalloc(&t->count, sizeof(unsigned int));
         if (e == hipSuccess) e = hipHostMalloc(&t->host_count, sizeof(unsigned int), hipHostMallocDefault);
@@ -191,6 +239,8 @@ int mhd_tape_forward(mhd_tape* t, double* dts, double* chs) {
     t->dts.assign(t->nsteps, 0.0);
     t->chs.assign(t->nsteps, 0.0);
     const int s_last = (t->nck - 1) * t->K;  // first step of the last segment
+    const std::vector<double*>& S = t->set[0].S;
+    const std::vector<double*>& T = t->set[0].T;
     for (int s = 0; s < t->nsteps; ++s) {
         if (s % t->K == 0)
             MHD_HIP(hipMemcpyAsync(t->ck[s / t->K], r->U[0], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
@@ -204,11 +254,11 @@ int mhd_tape_forward(mhd_tape* t, double* dts, double* chs) {
             // with the arithmetic of mhd_compute_dt / mhd_advance, so it needs no recompute.
             const int q = s - s_last;
             if (q == 0)
-                MHD_HIP(hipMemcpyAsync(t->S[0], r->U[0], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
+                MHD_HIP(hipMemcpyAsync(S[0], r->U[0], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
             const bool reuse = q == 0 ? r->acc_valid : mhd::fused_cfl();
-            if (int rc = mhd::dt_from_state(r, t->S[q], reuse, &dt)) return rc;
+            if (int rc = mhd::dt_from_state(r, S[q], reuse, &dt)) return rc;
             if (!std::isfinite(dt) || dt <= 0.0) return fail(MHD_ERR_DT, "dt non-finite or <= 0");
-            if (int rc = mhd::step_grids(r, t->S[q], t->T[q], t->S[q + 1], dt)) return rc;
+            if (int rc = mhd::step_grids(r, S[q], T[q], S[q + 1], dt)) return rc;
         }
         t->dts[s] = dt;
         t->chs[s] = r->ch;
@@ -216,7 +266,7 @@ int mhd_tape_forward(mhd_tape* t, double* dts, double* chs) {
         if (chs) chs[s] = r->ch;
     }
     const int m = t->nsteps - s_last;
-    MHD_HIP(hipMemcpyAsync(r->U[0], t->S[m], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
+    MHD_HIP(hipMemcpyAsync(r->U[0], S[m], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
     MHD_HIP(hipStreamSynchronize(r->stream()));
     r->acc_valid = mhd::fused_cfl();  // acc holds the CFL maximum of S[m], now U[0]
     t->recorded = true;
@@ -239,27 +289,65 @@ int mhd_tape_backward(mhd_tape* t, const double* lam_final, double* lam0, int la
     if (int rc = upload(r, lam_final, layout, t->lam[0])) return rc;
     int rc = MHD_OK;
     const bool kept = t->last_kept;
-    t->last_kept = t->nck == 1;  // other segments overwrite the segment buffers
-    for (int k = t->nck - 1; k >= 0 && !rc; --k) {
+    const bool two = t->two;
+    hipStream_t M = r->stream(), R = two ? t->side : M;
+    // Segment k lives in set (nck-1-k) % 2 (with one set, set 0): the last segment in
+    // set 0, where the forward left it.
+    auto set_of = [&](int k) { return two ? (t->nck - 1 - k) % 2 : 0; };
+    bool freed[2] = {false, false};  // ev_free[b] recorded: reverse steps read set b
+    auto recompute = [&](int k) -> int {
+        const int b = set_of(k);
+        const std::vector<double*>& S = t->set[b].S;
+        const std::vector<double*>& T = t->set[b].T;
         const int s0 = k * t->K, m = std::min(t->K, t->nsteps - s0);
-        if (!(kept && k == t->nck - 1)) {
-            MHD_HIP(hipMemcpyAsync(t->S[0], t->ck[k], grid_bytes(r), hipMemcpyDeviceToDevice, r->stream()));
-            // The reverse steps read U and U* of each step; the state after the segment
-            // is only needed for the last segment's finiteness check, so elsewhere the
-            // last step stops after its x sweep.
-            for (int q = 0; q < m && !rc; ++q) {
-                r->ch = t->chs[s0 + q];
-                const bool x_only = q == m - 1 && k != t->nck - 1;
-                rc = mhd::step_grids(r, t->S[q], t->T[q], t->S[q + 1], t->dts[s0 + q], false, x_only);
-            }
+        if (two && freed[b]) MHD_HIP(hipStreamWaitEvent(R, t->ev_free[b], 0));
+        MHD_HIP(hipMemcpyAsync(S[0], t->ck[k], grid_bytes(r), hipMemcpyDeviceToDevice, R));
+        // The reverse steps read U and U* of each step; the state after the segment is
+        // only needed for the last segment's finiteness check, so elsewhere the last
+        // step stops after its x sweep.
+        for (int q = 0; q < m; ++q) {
+            r->ch = t->chs[s0 + q];
+            const bool x_only = q == m - 1 && k != t->nck - 1;
+            if (int e = mhd::step_grids(r, S[q], T[q], S[q + 1], t->dts[s0 + q], false, x_only, R)) return e;
         }
+        if (two) MHD_HIP(hipEventRecord(t->ev_ready[b], R));
+        return MHD_OK;
+    };
+    if (two) {  // the recompute stream starts after what the run's stream holds
+        MHD_HIP(hipEventRecord(t->ev_join, M));
+        MHD_HIP(hipStreamWaitEvent(R, t->ev_join, 0));
+    }
+    t->last_kept = false;
+    for (int k = t->nck - 1; k >= 0 && !rc; --k) {
+        const int s0 = k * t->K, m = std::min(t->K, t->nsteps - s0), b = set_of(k);
+        const bool here = kept && k == t->nck - 1;
+        if (!here && (!two || k == t->nck - 1)) rc = recompute(k);
+        if (!rc && two && k > 0) rc = recompute(k - 1);  // overlaps the reverse steps of k
         if (rc) break;
+        if (two && !here) {
+            const hipError_t e = hipStreamWaitEvent(M, t->ev_ready[b], 0);
+            if (e != hipSuccess) { rc = mhd::hip_fail(e, "hipStreamWaitEvent"); break; }
+        }
+        const std::vector<double*>& S = t->set[b].S;
+        const std::vector<double*>& T = t->set[b].T;
         if (k == t->nck - 1) {
-            if ((rc = nonfinite(t, t->S[m], &bad))) break;
+            if ((rc = nonfinite(t, S[m], &bad))) break;
             if (bad) { rc = fail(MHD_ERR_INVALID, "adjoint: the recorded trajectory is not finite"); break; }
         }
         for (int q = m - 1; q >= 0 && !rc; --q)
-            rc = reverse_step(r, t->S[q], t->T[q], t->dts[s0 + q], t->chs[s0 + q], t->lam[0], t->lam[1]);
+            rc = reverse_step(r, S[q], T[q], t->dts[s0 + q], t->chs[s0 + q], t->lam[0], t->lam[1]);
+        if (!rc && two) {
+            const hipError_t e = hipEventRecord(t->ev_free[b], M);
+            if (e != hipSuccess) rc = mhd::hip_fail(e, "hipEventRecord");
+            freed[b] = true;
+        }
+    }
+    // Set 0 still holds the last segment when no other segment went through it.
+    if (!rc) t->last_kept = t->nck == 1 || (two && t->nck == 2);
+    if (two) {  // whatever the recompute stream still holds precedes the run's next work
+        const hipError_t e1 = hipEventRecord(t->ev_join, R);
+        const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(M, t->ev_join, 0) : e1;
+        if (e2 != hipSuccess && !rc) rc = mhd::hip_fail(e2, "tape stream join");
     }
     // The recompute writes only the segment buffers and skips the CFL reduction, so the
     // run's U[0], accumulator and pinned copy are untouched: restore the flags as well.

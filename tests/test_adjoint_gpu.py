@@ -97,9 +97,13 @@ def _oracle_chain(case, U0, nsteps, solver, limiter):
                                                  (2, 0, (2, 2, 0, 0), 1), (1, 1, (0, 0, 0, 0), 4),
                                                  (3, 0, (1, 1, 2, 2), 6), (2, 1, (3, 1, 3, 2), 5),
                                                  (2, 0, (3, 3, 3, 3), 4)])
-def test_tape_matches_host_chain_and_tangent(solver, limiter, bc, K):
+@pytest.mark.parametrize("sets", ["2", "1"])
+def test_tape_matches_host_chain_and_tangent(solver, limiter, bc, K, sets, monkeypatch):
+    """sets: the tape's segment buffer sets (2: the next segment is recomputed on a
+    second stream during the current one's reverse steps; 1: one after the other)."""
     from mhd_gpu_acc_amd.adjoint import Tape
-    say(f"[gpu] tape 40x36 solver={solver} limiter={limiter} bc={bc} K={K}")
+    monkeypatch.setenv("MHD_TAPE_SETS", sets)
+    say(f"[gpu] tape 40x36 solver={solver} limiter={limiter} bc={bc} K={K} sets={sets}")
     nsteps = 12
     case = small_case(40, 36, bc)
     U0 = perturbed_ot(case, 31)
