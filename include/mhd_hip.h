@@ -234,7 +234,10 @@ int mhd_batch_plan_ot_family(uint64_t seed, int n, int* sizes, mhd_ic_params* pa
  * their adjoint back to it (boundary_cpu.hpp:12-32, solver_cpu.cpp:396-398,434-436).  The
  * gradient is with respect to every value of U_0, ghost cells included. */
 typedef struct mhd_tape mhd_tape;
-/* Device memory: ceil(nsteps/K) checkpoints, 2K+1 segment grids, 2 adjoint grids. */
+/* Device memory: ceil(nsteps/K) checkpoints, 2K+1 segment grids, 2 adjoint grids; with
+ * two or more segments a second set of 2K+1 segment grids and a second stream, so that
+ * the backward recomputes the next segment while it reverses the current one (one set
+ * when the second does not fit, or with MHD_TAPE_SETS=1 in the environment). */
 int mhd_tape_create(mhd_run* run, int nsteps, int K, mhd_tape** out);
 int mhd_tape_destroy(mhd_tape* tape);
 /* Advances the run nsteps steps from its current state (as mhd_step_to, without
@@ -243,7 +246,8 @@ int mhd_tape_destroy(mhd_tape* tape);
 int mhd_tape_forward(mhd_tape* tape, double* dts, double* chs);
 /* lam_final = dL/dU_N in, lam0 = dL/dU_0 out, both in an mhd_set_state layout.
  * Recomputes each K-step segment from its checkpoint, then runs the reverse
- * steps.  MHD_ERR_INVALID when the recorded trajectory is not finite. */
+ * steps (both ordered after the run's stream, and the run's stream after them).
+ * MHD_ERR_INVALID when the recorded trajectory is not finite. */
 int mhd_tape_backward(mhd_tape* tape, const double* lam_final, double* lam0, int layout);
 /* One reverse step over caller device grids: Uold a step's input, Utmp its
  * intermediate U* (ghosts filled), lam = dL/d(step output) in and dL/dUold out;
