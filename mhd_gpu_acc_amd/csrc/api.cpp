@@ -124,6 +124,7 @@ int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, doubl
     a.xwg = r->xwg;
     a.ywg = r->ywg;
     a.xgrid = r->xgrid;
+    a.xtail = r->xtail;
     a.addr_buf = r->addr_buf;
     a.cus = r->cus;
     a.frame = r->frame;
@@ -194,6 +195,8 @@ int mhd_run_create(const mhd_run_desc* desc, mhd_run** out) {
     r->ywg = ywg ? std::atoi(ywg) : 0;
     const char* xgrid = std::getenv("MHD_XGRID");
     r->xgrid = xgrid ? std::atoi(xgrid) : 8;  // persistent x sweep, 8 blocks per CU (measured best)
+    const char* xtail = std::getenv("MHD_XTAIL");  // per mille of x segments in one-segment blocks
+    r->xtail = xtail ? std::min(std::max(std::atoi(xtail), 0), 1000) : 50;  // 50: measured best at 4096^2
     const char* addr = std::getenv("MHD_ADDR");  // "flat": the sweeps' path for grids above 4 GiB
     r->addr_buf = !(addr && std::strcmp(addr, "flat") == 0);
     int dev = 0;

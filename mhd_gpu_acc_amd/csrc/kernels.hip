@@ -37,10 +37,14 @@ namespace mhd {
 #define MHD_X_ATTR
 #endif
 
+// Segments 0 .. npers-1 go to the first gp blocks, block b taking b, b+gp, ... (loading
+// each next segment while it computes the current one); every further block takes one of
+// the remaining segments.  Blocks are dispatched in order, so those one-segment blocks
+// fill the chip while the last rounds of the looping blocks finish.
 template <int SOLVER, int LIMITER, int WG, bool BUF>
 __global__ __launch_bounds__(WG) MHD_X_ATTR void xsweep_kernel(const double* __restrict__ Uo, double* __restrict__ Ut, Geom g,
                                                     Phys ph, double dtd, double hdtd,
-                                                    unsigned long long* __restrict__ acc, int nseg) {
+                                                    unsigned long long* __restrict__ acc, int gp, int npers) {
     __shared__ XLds<WG> L;
     const int t = threadIdx.x;
     if (acc && blockIdx.x == 0 && t == 0) *acc = 0ull;  // CFL accumulator for this step's ysweep
@@ -48,17 +52,19 @@ __global__ __launch_bounds__(WG) MHD_X_ATTR void xsweep_kernel(const double* __r
     const std::conditional_t<BUF, BufPlanes<true>, Planes<true, double>> dst(Ut, g.ps);
     const int last = g.tnx * g.tny - 1;
     const int kb = NG * g.tnx - 2 + t;
-    int s = blockIdx.x;
-    if (s >= nseg) return;
+    const bool looping = (int)blockIdx.x < gp;
+    int s = looping ? (int)blockIdx.x : npers + ((int)blockIdx.x - gp);
+    const int end = looping ? npers : s + 1;
+    if (s >= end) return;
     int k = kb + s * (WG - 4);
     Cons un = src.load(min(k, last));  // clamp: tail lanes read valid memory
     for (;;) {
-        const int sn = s + gridDim.x;
+        const int sn = s + gp;
         const int kn = kb + sn * (WG - 4);
         const Cons u = un;  // the segment loaded one iteration ago
-        if (sn < nseg) un = src.load(min(kn, last));
+        if (sn < end) un = src.load(min(kn, last));
         xsweep_segment<SOLVER, LIMITER, WG>(u, k, t, dst, g, ph, dtd, hdtd, L);
-        if (sn >= nseg) break;
+        if (sn >= end) break;
         lds_barrier();  // sw is rewritten by the next segment
         s = sn;
         k = kn;
@@ -146,10 +152,13 @@ template <int S, int L, int WG, bool BUF>
 static void launch_x(const StepArgs& a, const Geom& g, const Phys& ph, double dtdx) {
     const long long flat = (long long)g.tnx * g.ny;
     const int nseg = (int)((flat + (WG - 4) - 1) / (WG - 4));
-    // xgrid > 0: persistent grid of xgrid blocks per CU; 0: one block per segment
-    const int nb = a.xgrid > 0 ? min(nseg, a.xgrid * a.cus) : nseg;
-    hipLaunchKernelGGL((xsweep_kernel<S, L, WG, BUF>), dim3((unsigned)nb), dim3(WG), 0, a.stream, a.Uold, a.Utmp, g,
-                       ph, dtdx, 0.5 * dtdx, a.acc, nseg);
+    // xgrid > 0: persistent grid of xgrid blocks per CU, plus xtail per mille of the
+    // segments in one-segment blocks after it; 0: one block per segment
+    const int gp = a.xgrid > 0 ? min(nseg, a.xgrid * a.cus) : nseg;
+    const int tail = a.xgrid > 0 ? (int)((long long)nseg * a.xtail / 1000) : 0;
+    const int npers = nseg - tail;
+    hipLaunchKernelGGL((xsweep_kernel<S, L, WG, BUF>), dim3((unsigned)(gp + tail)), dim3(WG), 0, a.stream, a.Uold,
+                       a.Utmp, g, ph, dtdx, 0.5 * dtdx, a.acc, gp, npers);
 }
 
 template <int S, int L>

@@ -22,6 +22,8 @@ struct StepArgs {
     int xwg;                 // x-sweep workgroup width: 64, 128 or 256 (0 = default)
     int ywg;                 // y-sweep workgroup width: 64, 128 or 256 (0 = default)
     int xgrid;               // x-sweep blocks per CU (persistent), 0 = one block per segment
+    int xtail;               // per mille of the x sweep's segments in one-segment blocks after
+                             // the persistent ones (with xgrid > 0)
     bool addr_buf;           // sweeps address through buffer resources where the grid allows (else flat)
     int cus;                 // compute units of the device
     double* frame;           // undamped psi of the interior frame (2*2*(nx+ny) doubles)

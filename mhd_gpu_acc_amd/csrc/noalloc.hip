@@ -98,6 +98,7 @@ extern "C" int mhd_advance_grids_noalloc(const mhd_run_desc* d, double ch, const
     a.xwg = 0;
     a.ywg = 0;
     a.xgrid = 8;  // the run handle's default launch shapes
+    a.xtail = 50;
     a.addr_buf = true;
     a.cus = s.cus;
     a.frame = reinterpret_cast<double*>(s.ptr + SCRATCH_ACC_BYTES);

@@ -27,6 +27,7 @@ struct mhd_run {
     int xwg = 0;
     int ywg = 0;
     int xgrid = 0;
+    int xtail = 50;  // per mille of x segments in one-segment blocks (MHD_XTAIL)
     bool addr_buf = true;
     int cus = 1;
     bool profiling = false;

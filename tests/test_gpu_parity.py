@@ -167,10 +167,13 @@ def test_all_bc_mixes_step_vs_oracle(bc):
                                    dict(MHD_XWG=64), dict(MHD_XWG=128), dict(MHD_XGRID=1), dict(MHD_XGRID=3),
                                    dict(MHD_XWG=64, MHD_XGRID=2), dict(MHD_YWG=64), dict(MHD_YWG=128, MHD_YSEG=5),
                                    dict(MHD_ADDR="flat"), dict(MHD_ADDR="flat", MHD_YSEG=7),
-                                   dict(MHD_ADDR="flat", MHD_XWG=128, MHD_XGRID=0)])
+                                   dict(MHD_ADDR="flat", MHD_XWG=128, MHD_XGRID=0), dict(MHD_XTAIL=0),
+                                   dict(MHD_XTAIL=300), dict(MHD_XTAIL=1000, MHD_XGRID=1),
+                                   dict(MHD_XTAIL=77, MHD_XWG=64, MHD_XGRID=2)])
 def test_launch_shape_independence(knobs, monkeypatch):
-    """Workgroup widths, persistent grids, segment lengths and the sweeps' addressing (flat,
-    as for grids above 4 GiB, or buffer resources) change the launch, never the result."""
+    """Workgroup widths, persistent grids, the share of x segments in one-segment blocks,
+    segment lengths and the sweeps' addressing (flat, as for grids above 4 GiB, or buffer
+    resources) change the launch, never the result."""
     for k, v in knobs.items():
         monkeypatch.setenv(k, str(v))
     cfg = m.get_n_case_config("orszag_tang_unit", 1)
