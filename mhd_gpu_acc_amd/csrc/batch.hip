@@ -20,6 +20,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -61,7 +62,7 @@ inline Phys phys_of(double gamma, double ch) { return Phys{gamma, gamma - 1.0, c
 template <int SOLVER, int LIMITER, int WG, bool BUF>
 __global__ __launch_bounds__(WG) void batch_xsweep_kernel(const BatchRun* __restrict__ runs,
                                                           const BatchStep* __restrict__ steps,
-                                                          const int2* __restrict__ tiles, int ntiles,
+                                                          const int2* __restrict__ tiles, int gp, int npers,
                                                           unsigned long long* __restrict__ acc, int nruns,
                                                           unsigned int* __restrict__ queue) {
     __shared__ XLds<WG> L;
@@ -70,10 +71,15 @@ __global__ __launch_bounds__(WG) void batch_xsweep_kernel(const BatchRun* __rest
         for (int r = t; r < nruns; r += WG) acc[r] = 0ull;  // CFL maxima of this step's y sweep
         if (t == 0) *queue = 0u;                             // y-sweep work counter
     }
-    int q = blockIdx.x;
-    if (q >= ntiles) return;
-    // The cells of tile q + gridDim.x are loaded before tile q is computed, so their
-    // load latency overlaps the arithmetic (as the plain x sweep's segments).
+    // Tiles 0 .. npers-1 go to the first gp blocks (block b: b, b+gp, ...); every further
+    // block takes one of the remaining tiles and, dispatched after the others, fills the
+    // chip while their last round ends (as the plain x sweep).
+    const bool looping = (int)blockIdx.x < gp;
+    int q = looping ? (int)blockIdx.x : npers + ((int)blockIdx.x - gp);
+    const int end = looping ? npers : q + 1;
+    if (q >= end) return;
+    // The cells of tile q + gp are loaded before tile q is computed, so their load
+    // latency overlaps the arithmetic (as the plain x sweep's segments).
     const auto load_tile = [&](int2 tl) {
         const BatchRun& R = runs[tl.x];
         const BatchStep& P = steps[tl.x];
@@ -86,10 +92,10 @@ __global__ __launch_bounds__(WG) void batch_xsweep_kernel(const BatchRun* __rest
     int2 tl = tiles[q];
     Cons un = load_tile(tl);
     for (;;) {
-        const int qn = q + (int)gridDim.x;
+        const int qn = q + gp;
         const Cons u = un;
         int2 tln = tl;
-        if (qn < ntiles) {
+        if (qn < end) {
             tln = tiles[qn];
             un = load_tile(tln);
         }
@@ -101,7 +107,7 @@ __global__ __launch_bounds__(WG) void batch_xsweep_kernel(const BatchRun* __rest
         const Dst dst = plane_view<BUF, Dst>(R.U[1], g.ps);
         const int k = NG * g.tnx - 2 + tl.y * (WG - 4) + t;
         xsweep_segment<SOLVER, LIMITER, WG>(u, k, t, dst, g, ph, P.dtdx, P.hdtdx, L);
-        if (qn >= ntiles) break;
+        if (qn >= end) break;
         lds_barrier();  // LDS is rewritten by the next tile
         q = qn;
         tl = tln;
@@ -218,6 +224,7 @@ struct mhd_batch {
     bool buf = true;  // every grid within BUF_MAX_BYTES: buffer-resource addressing
     // launch shape (autotuned)
     int xwg = 256, xgrid = 8, yseg = 64, ygrid = 8;  // yseg: tall tile height; tails use yseg/4
+    int xtail = 50;  // per mille of the x tiles in one-tile blocks after the persistent ones
     unsigned int* d_queue = nullptr;
     int2* d_xt = nullptr;  int nxt = 0;
     int4* d_yt = nullptr;  int nyt = 0;
@@ -288,9 +295,10 @@ int build_tiles(mhd_batch* b) {
 }
 
 template <int S, int L, int WG, bool BUF> void launch_bx(mhd_batch* b) {
-    const int nb = std::min(b->nxt, b->xgrid * b->cus);
-    hipLaunchKernelGGL((batch_xsweep_kernel<S, L, WG, BUF>), dim3(std::max(nb, 1)), dim3(WG), 0, b->stream,
-                       b->d_runs, b->d_steps, b->d_xt, b->nxt, b->d_acc, b->n, b->d_queue);
+    const int gp = std::max(std::min(b->nxt, b->xgrid * b->cus), 1);
+    const int tail = (int)((long long)b->nxt * b->xtail / 1000);
+    hipLaunchKernelGGL((batch_xsweep_kernel<S, L, WG, BUF>), dim3(gp + tail), dim3(WG), 0, b->stream, b->d_runs,
+                       b->d_steps, b->d_xt, gp, b->nxt - tail, b->d_acc, b->n, b->d_queue);
 }
 
 template <int S, int L, bool BUF> void launch_by(mhd_batch* b) {
@@ -462,6 +470,8 @@ int mhd_batch_create(const mhd_run_desc* descs, int n, mhd_batch** out) {
         b->own_stream = true;
     }
     b->runs.resize(n);
+    const char* xtail = std::getenv("MHD_XTAIL");  // as for a run (api.cpp)
+    if (xtail) b->xtail = std::min(std::max(std::atoi(xtail), 0), 1000);
     const char* addr = std::getenv("MHD_ADDR");  // "flat": the global-address-space views
     b->buf = !(addr && std::strcmp(addr, "flat") == 0);
     for (int r = 0; r < n; ++r) {

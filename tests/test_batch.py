@@ -140,10 +140,16 @@ def test_batch_tuning_keeps_results():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape", [(128, 1, 1, 1), (256, 3, 7, 2), (128, 16, 64, 16)])
-def test_batch_launch_shapes(shape):
+@pytest.mark.parametrize("shape,xtail", [((128, 1, 1, 1), None), ((256, 3, 7, 2), None),
+                                         ((128, 16, 64, 16), None), ((256, 1, 16, 2), 0),
+                                         ((256, 2, 16, 2), 400), ((128, 1, 32, 4), 1000)])
+def test_batch_launch_shapes(shape, xtail, monkeypatch):
+    """Launch shapes and the share of x tiles in one-tile blocks (MHD_XTAIL, per mille)
+    change the launch, never the result."""
+    if xtail is not None:
+        monkeypatch.setenv("MHD_XTAIL", str(xtail))
     cfgs, states = _mixed_runs()
-    say(f"[gpu] batch shape {shape}")
+    say(f"[gpu] batch shape {shape} xtail {xtail}")
     with Batch(cfgs, 2, 1) as b:
         b.set_shape(*shape)
         for r, U in enumerate(states):
