@@ -183,12 +183,13 @@ static hipError_t launch_step_sl(const StepArgs& a) {
     if (a.x_only) return hipSuccess;
     if (a.ev && (e = hipEventRecord(a.ev[2], a.stream)) != hipSuccess) return e;
     // Rows per lane segment: enough column-waves x segments for 2 waves per SIMD
-    // (8 per CU), at most 32 rows (the march's 2-row halo is then 6 %) and at least 4.
+    // (8 per CU), at most 64 rows (the march's 2-row halo is then 3 %; 64 rows took 0.5 %
+    // off the 4096^2 y sweep against 32) and at least 4.
     int seg = a.seg;
     if (seg <= 0) {
         const long long colwaves = (g.nx + 63) / 64;
         const long long rows = colwaves * g.ny / (8LL * (a.cus > 0 ? a.cus : 1));
-        seg = (int)(rows < 4 ? 4 : (rows > 32 ? 32 : rows));
+        seg = (int)(rows < 4 ? 4 : (rows > 64 ? 64 : rows));
     }
     const int ywg = (a.ywg == 64 || a.ywg == 128) ? a.ywg : 256;
     dim3 gy((unsigned)((g.nx + ywg - 1) / ywg), (unsigned)((g.ny + seg - 1) / seg));
