@@ -44,6 +44,7 @@ struct Cons {
 struct Prim {
     double r, un, ut, uz, bn, bt, bz, p, psi;
     double b2;
+    double rr;  // on the device, the refined reciprocal of r that a / r forms (see FastDiv)
 };
 
 // Per-launch physics constants.
@@ -59,9 +60,128 @@ struct Phys {
 MHD_DI bool finite(double x) { return __builtin_isfinite(x); }MHD_DI double std_min(double a, double b) { return (b < a) ? b : a; }  // std::min
 MHD_DI double std_max(double a, double b) { return (a < b) ? b : a; }  // std::max
 
-// physics.hpp:69-78
-MHD_DI Prim to_prim(const Cons& u, const Phys& ph) {
-    const double inv = 1.0 / u.r;
+// ---------------------------------------------------------------------------------
+// Quotients by a shared denominator
+// ---------------------------------------------------------------------------------
+// The device lowers a / b (FP64) to: div_scale of b and of a, rcp of the scaled b, two
+// Newton-Raphson corrections of that reciprocal, the quotient, its residual (FMA),
+// div_fmas and div_fixup.  div_scale leaves both operands unchanged, and div_fmas is
+// then a plain FMA, whenever b is normal with a normal reciprocal, a is zero or at
+// least 2^-969 in magnitude, the quotient is normal and a's biased exponent exceeds
+// b's by less than 768.  div_fixup returns that quotient for finite non-zero operands
+// and the IEEE result for a zero, infinite or NaN operand whatever quotient it is given.
+// So for operands whose frexp exponents lie in [-DIV_EXP, DIV_EXP] (|x| in
+// [2^-381, 2^380): every condition above holds with room), or that are zero, infinite
+// or NaN (frexp exponent 0), the sequence without the two div_scale steps returns the
+// bits of a / b.  The refined reciprocal depends on b alone, so divisions by one
+// denominator share it: each further quotient is a multiply, two FMAs and the fixup,
+// instead of the whole sequence around a quarter-rate rcp and its VCC-carried scale.
+//
+// FastDiv forms quotients that way and keeps the smallest and largest frexp exponent
+// of every operand it divided; ExactDiv divides.  A caller evaluates with FastDiv and
+// keeps the result when no lane of the wave saw an exponent outside the range (one
+// ballot), else evaluates again with ExactDiv: either way every quotient is a / b.
+constexpr int DIV_EXP = 380;
+
+struct Rcp {  // a denominator and its refined reciprocal (FastDiv) or 0 (ExactDiv)
+    double b, r;
+};
+
+// The reciprocal the device's division refines from rcp(b) (0 on the host).
+MHD_DI double refined_rcp(double b) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const double r0 = __builtin_amdgcn_rcp(b);
+    const double e0 = __builtin_fma(-b, r0, 1.0);
+    const double r1 = __builtin_fma(r0, e0, r0);
+    const double e1 = __builtin_fma(-b, r1, 1.0);
+    return __builtin_fma(r1, e1, r1);
+#else
+    (void)b;
+    return 0.0;
+#endif
+}
+
+struct ExactDiv {
+    // The reciprocal is formed for Prim::rr only; unused, the compiler drops it.
+    MHD_DI Rcp rcp(double b) { return Rcp{b, refined_rcp(b)}; }
+    MHD_DI Rcp given(double b, double r) { return Rcp{b, r}; }
+    MHD_DI double q(double a, const Rcp& d) { return a / d.b; }
+};
+
+struct FastDiv {
+    int lo = 0, hi = 0;  // frexp exponents seen (0 for zero, infinite and NaN operands)
+    MHD_DI void note(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+        const int e = __builtin_amdgcn_frexp_exp(x);
+        lo = e < lo ? e : lo;
+        hi = e > hi ? e : hi;
+#else
+        (void)x;
+#endif
+    }
+    MHD_DI Rcp rcp(double b) {
+        note(b);
+        return Rcp{b, refined_rcp(b)};
+    }
+    // A reciprocal formed earlier by refined_rcp(b) (Prim::rr).
+    MHD_DI Rcp given(double b, double r) {
+        note(b);
+        return Rcp{b, r};
+    }
+    MHD_DI double q(double a, const Rcp& d) {
+        note(a);
+#ifdef __HIP_DEVICE_COMPILE__
+        const double q0 = a * d.r;
+        const double e = __builtin_fma(-d.b, q0, a);
+        return __builtin_amdgcn_div_fixup(__builtin_fma(e, d.r, q0), d.b, a);
+#else
+        return a / d.b;
+#endif
+    }
+    // Every active lane's operands were in range (wave-uniform).
+    MHD_DI bool wave_ok() const {
+#ifdef __HIP_DEVICE_COMPILE__
+        return __builtin_amdgcn_ballot_w64(lo < -DIV_EXP || hi > DIV_EXP) == 0;
+#else
+        return true;
+#endif
+    }
+};
+
+// FastDiv's sequence for operands the caller has already found in range.
+struct CoreDiv {
+    MHD_DI Rcp rcp(double b) { return Rcp{b, refined_rcp(b)}; }
+    MHD_DI Rcp given(double b, double r) { return Rcp{b, r}; }
+    MHD_DI double q(double a, const Rcp& d) {
+#ifdef __HIP_DEVICE_COMPILE__
+        const double q0 = a * d.r;
+        const double e = __builtin_fma(-d.b, q0, a);
+        return __builtin_amdgcn_div_fixup(__builtin_fma(e, d.r, q0), d.b, a);
+#else
+        return a / d.b;
+#endif
+    }
+};
+
+// Whether every active lane's values have frexp exponents in [-DIV_EXP, DIV_EXP] (or are
+// zero, infinite or NaN), so that CoreDiv may divide by and into them (wave-uniform;
+// false on the host).
+MHD_DI bool wave_in_range(double x0, double x1, double x2 = 0.0, double x3 = 0.0, double x4 = 0.0) {
+#ifdef __HIP_DEVICE_COMPILE__
+    FastDiv g;
+    g.note(x0); g.note(x1); g.note(x2); g.note(x3); g.note(x4);
+    return g.wave_ok();
+#else
+    (void)x0; (void)x1; (void)x2; (void)x3; (void)x4;
+    return false;
+#endif
+}
+
+// physics.hpp:69-78.  rr keeps the refined reciprocal of rho for later quotients by
+// rho (fast_speed under FastDiv).
+template <class D> MHD_DI Prim to_prim(const Cons& u, const Phys& ph, D& dv) {
+    const Rcp r = dv.rcp(u.r);
+    const double inv = dv.q(1.0, r);
     Prim w;
     w.r = u.r;
     w.un = u.mn * inv;
@@ -75,11 +195,16 @@ MHD_DI Prim to_prim(const Cons& u, const Phys& ph) {
     w.p = ph.gm1 * (u.e - ke - 0.5 * b2);
     w.psi = u.psi;
     w.b2 = b2;
+    w.rr = r.r;
     return w;
 }
+MHD_DI Prim to_prim(const Cons& u, const Phys& ph) {
+    ExactDiv ex;
+    return to_prim(u, ph, ex);
+}
 
-// physics.hpp:83-89
-MHD_DI Cons to_cons(const Prim& w, const Phys& ph) {
+// physics.hpp:83-89.  rg: the reciprocal of gamma - 1 (rcp(ph.gm1) of the same policy).
+template <class D> MHD_DI Cons to_cons(const Prim& w, const Phys& ph, D& dv, const Rcp& rg) {
     const double b2 = w.bn * w.bn + w.bt * w.bt + w.bz * w.bz;
     const double ke = 0.5 * w.r * (w.un * w.un + w.ut * w.ut + w.uz * w.uz);
     Cons u;
@@ -90,9 +215,13 @@ MHD_DI Cons to_cons(const Prim& w, const Phys& ph) {
     u.bn = w.bn;
     u.bt = w.bt;
     u.bz = w.bz;
-    u.e = w.p / ph.gm1 + ke + 0.5 * b2;
+    u.e = dv.q(w.p, rg) + ke + 0.5 * b2;
     u.psi = w.psi;
     return u;
+}
+MHD_DI Cons to_cons(const Prim& w, const Phys& ph) {
+    ExactDiv ex;
+    return to_cons(w, ph, ex, ex.rcp(ph.gm1));
 }
 
 // physics.hpp:105-164.  `w` must be to_prim(u): the reference recomputes the same
@@ -138,13 +267,19 @@ MHD_DI double sqrt_cr(double x) {
     return sqrt(x);
 }
 
-// physics.hpp:175-190 (fast magnetosonic speed along the normal)
-MHD_DI double fast_speed(const Prim& w, const Phys& ph) {
-    const double a2 = ph.gamma * w.p / w.r;
-    const double b2 = w.b2 / w.r;
-    const double bn2 = w.bn * w.bn / w.r;
+// physics.hpp:175-190 (fast magnetosonic speed along the normal); under FastDiv the
+// three divisions by rho use the reciprocal to_prim formed (w must come from to_prim).
+template <class D> MHD_DI double fast_speed(const Prim& w, const Phys& ph, D& dv) {
+    const Rcp r = dv.rcp(w.r);
+    const double a2 = dv.q(ph.gamma * w.p, r);
+    const double b2 = dv.q(w.b2, r);
+    const double bn2 = dv.q(w.bn * w.bn, r);
     const double disc = (a2 + b2) * (a2 + b2) - 4.0 * a2 * bn2;
     return sqrt_cr(0.5 * (a2 + b2 + sqrt_cr(disc > 0.0 ? disc : 0.0)));
+}
+MHD_DI double fast_speed(const Prim& w, const Phys& ph) {
+    ExactDiv ex;
+    return fast_speed(w, ph, ex);
 }
 
 // riemann.hpp:54-57
@@ -180,11 +315,12 @@ MHD_DI Cons fan(const Cons& f, double s, const Cons& a, const Cons& b) {
 // ---------------------------------------------------------------------------------
 // HLL (riemann.hpp:70-107).  FL/FR and the prims are passed in.
 // ---------------------------------------------------------------------------------
+template <class D>
 MHD_DI Cons hll(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur, const Prim& vr, const Cons& fr,
-                const Phys& ph) {
+                const Phys& ph, D& dv) {
     if (!prim_physical(vl) || !prim_physical(vr)) return half_sum(fl, fr);
-    const double cfl = fast_speed(vl, ph);
-    const double cfr = fast_speed(vr, ph);
+    const double cfl = fast_speed(vl, ph, dv);
+    const double cfr = fast_speed(vr, ph, dv);
     const double sl = fmin(fmin(vl.un - cfl, vr.un - cfr), -ph.ch);
     const double sr = fmax(fmax(vl.un + cfl, vr.un + cfr), ph.ch);
     if (sl >= 0.0) return fl;
@@ -203,6 +339,11 @@ MHD_DI Cons hll(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur, 
                 inv * (sr * fl.e - sl * fr.e + ss * (ur.e - ul.e)),
                 inv * (sr * fl.psi - sl * fr.psi + ss * (ur.psi - ul.psi))};
 }
+MHD_DI Cons hll(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur, const Prim& vr, const Cons& fr,
+                const Phys& ph) {
+    ExactDiv ex;
+    return hll(ul, vl, fl, ur, vr, fr, ph, ex);
+}
 
 // ---------------------------------------------------------------------------------
 // HLLC (riemann.hpp:122-241)
@@ -211,10 +352,11 @@ MHD_DI Cons hll(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur, 
 // 194-197); otherwise f is the flux.  Only the physical flux of the side the
 // interface lies on is formed (FL for S_M >= 0 or S_L >= 0, FR otherwise), which
 // is the one the reference's region selection returns or builds on.
-MHD_DI bool hllc(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr, const Phys& ph, Cons& f) {
+template <class D>
+MHD_DI bool hllc(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr, const Phys& ph, Cons& f, D& dv) {
     if (!prim_physical(vl) || !prim_physical(vr)) return false;
-    const double cfl = fast_speed(vl, ph);
-    const double cfr = fast_speed(vr, ph);
+    const double cfl = fast_speed(vl, ph, dv);
+    const double cfr = fast_speed(vr, ph, dv);
     const double sl = fmin(vl.un - cfl, vr.un - cfr);
     const double sr = fmax(vl.un + cfl, vr.un + cfr);
     if (sl >= 0.0) { f = flux(ul, vl, ph); return true; }
@@ -252,32 +394,40 @@ struct SideStar {
     double bu;                 // B.u of the single-star state
 };
 
-// L* / R* states (riemann.hpp:341-373).  s = S_L or S_R.
-MHD_DI SideStar side_star(const Cons& u, const Prim& v, double s, double sm, double ptl, double ptstar) {
+// L* / R* states (riemann.hpp:341-373).  s = S_L or S_R.  The four divisions by
+// the denominator d share one reciprocal under FastDiv.
+template <class D>
+MHD_DI SideStar side_star(const Cons& u, const Prim& v, double s, double sm, double ptl, double ptstar, D& dv) {
     SideStar o;
     const double d = v.r * (s - v.un) * (s - sm) - v.bn * v.bn;
     if (fabs(d) < 1.0e-14) {
         o.ut = v.ut; o.uw = v.uz; o.bt = v.bt; o.bw = v.bz;
     } else {
-        o.ut = v.ut - v.bn * v.bt * (sm - v.un) / d;
-        o.uw = v.uz - v.bn * v.bz * (sm - v.un) / d;
+        const Rcp rd = dv.rcp(d);
+        o.ut = v.ut - dv.q(v.bn * v.bt * (sm - v.un), rd);
+        o.uw = v.uz - dv.q(v.bn * v.bz * (sm - v.un), rd);
         const double k = v.r * (s - v.un) * (s - v.un) - v.bn * v.bn;
-        o.bt = v.bt * k / d;
-        o.bw = v.bz * k / d;
+        o.bt = dv.q(v.bt * k, rd);
+        o.bw = dv.q(v.bz * k, rd);
     }
     const double bu = v.bn * v.un + v.bt * v.ut + v.bz * v.uz;
     o.bu = v.bn * sm + o.bt * o.ut + o.bw * o.uw;
     o.e = (u.e * (s - v.un) - ptl * v.un + ptstar * sm + v.bn * (o.bu - bu)) / (s - sm);
     return o;
 }
+MHD_DI SideStar side_star(const Cons& u, const Prim& v, double s, double sm, double ptl, double ptstar) {
+    ExactDiv ex;
+    return side_star(u, v, s, sm, ptl, ptstar, ex);
+}
 
 // Returns false when the reference falls back to HLL (riemann.hpp:262-264,
 // 301-303, 316-318, 325-328); otherwise f is the flux.  As in hllc, only the
 // physical flux of the interface's side is formed.
-MHD_DI bool hlld(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr, const Phys& ph, Cons& f) {
+template <class D>
+MHD_DI bool hlld(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr, const Phys& ph, Cons& f, D& dv) {
     if (!prim_physical(vl) || !prim_physical(vr)) return false;
-    const double cfl = fast_speed(vl, ph);
-    const double cfr = fast_speed(vr, ph);
+    const double cfl = fast_speed(vl, ph, dv);
+    const double cfr = fast_speed(vr, ph, dv);
     const double sl = fmin(vl.un - cfl, vr.un - cfr);
     const double sr = fmax(vl.un + cfl, vr.un + cfr);
     if (fabs(sr - sl) < 1.0e-14) return false;
@@ -302,8 +452,8 @@ MHD_DI bool hlld(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr,
     const double sls = sm - fabs(vl.bn) / sql;
     const double srs = sm + fabs(vr.bn) / sqr;
     const double psim = 0.5 * (vl.psi + vr.psi) - ph.half_ch * (vr.bn - vl.bn);
-    const SideStar L = side_star(ul, vl, sl, sm, ptl, ptstar);
-    const SideStar R = side_star(ur, vr, sr, sm, ptr, ptstar);
+    const SideStar L = side_star(ul, vl, sl, sm, ptl, ptstar, dv);
+    const SideStar R = side_star(ur, vr, sr, sm, ptr, ptstar, dv);
     const bool left = sm >= 0.0;
     const bool outer = left ? (sls >= 0.0) : (srs <= 0.0);
     const Cons& us = left ? ul : ur;
@@ -340,11 +490,12 @@ MHD_DI bool hlld(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr,
 // ---------------------------------------------------------------------------------
 // FORCE (riemann.hpp:486-525)
 // ---------------------------------------------------------------------------------
+template <class D>
 MHD_DI Cons force(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur, const Prim& vr, const Cons& fr,
-                  const Phys& ph) {
+                  const Phys& ph, D& dv) {
     if (!prim_physical(vl) || !prim_physical(vr)) return half_sum(fl, fr);
-    const double cfl = fast_speed(vl, ph);
-    const double cfr = fast_speed(vr, ph);
+    const double cfl = fast_speed(vl, ph, dv);
+    const double cfr = fast_speed(vr, ph, dv);
     const double a = fmax(fmax(fabs(vl.un) + cfl, fabs(vr.un) + cfr), ph.ch);
     if (a < 1.0e-14) return half_sum(fl, fr);
     const double ha = 0.5 * a;
@@ -363,6 +514,11 @@ MHD_DI Cons force(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur
     if (!prim_physical(vri)) return flf;
     return half_sum(flf, flux(uri, vri, ph));
 }
+MHD_DI Cons force(const Cons& ul, const Prim& vl, const Cons& fl, const Cons& ur, const Prim& vr, const Cons& fr,
+                  const Phys& ph) {
+    ExactDiv ex;
+    return force(ul, vl, fl, ur, vr, fr, ph, ex);
+}
 
 // riemann.hpp:544-563: exact upwind flux of the (B_n, psi) subsystem.
 MHD_DI void glm_overwrite(Cons& f, const Cons& ul, const Cons& ur, const Phys& ph) {
@@ -375,28 +531,42 @@ MHD_DI void glm_overwrite(Cons& f, const Cons& ul, const Cons& ur, const Phys& p
 
 // riemann.hpp:568-592.  XDIR selects which rotated slot is the reference's
 // F.Bx in the finiteness test (:583-586): B_n on x sweeps, B_t on y sweeps.
-template <int SOLVER, bool XDIR>
-MHD_DI Cons riemann(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr, const Phys& ph) {
+template <int SOLVER, bool XDIR, class D>
+MHD_DI Cons riemann_with(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr, const Phys& ph, D& dv) {
     if (SOLVER == 0 || SOLVER == 3) {
         const Cons fl = flux(ul, vl, ph);
         const Cons fr = flux(ur, vr, ph);
-        if (SOLVER == 0) return hll(ul, vl, fl, ur, vr, fr, ph);
-        Cons f = force(ul, vl, fl, ur, vr, fr, ph);
+        if (SOLVER == 0) return hll(ul, vl, fl, ur, vr, fr, ph, dv);
+        Cons f = force(ul, vl, fl, ur, vr, fr, ph, dv);
         const double fbx = XDIR ? f.bn : f.bt;
-        if (!finite(f.r) || !finite(f.e) || !finite(fbx) || !finite(f.psi)) return hll(ul, vl, fl, ur, vr, fr, ph);
+        if (!finite(f.r) || !finite(f.e) || !finite(fbx) || !finite(f.psi)) return hll(ul, vl, fl, ur, vr, fr, ph, dv);
         glm_overwrite(f, ul, ur, ph);
         return f;
     }
     Cons f;
-    const bool ok = (SOLVER == 1) ? hllc(ul, vl, ur, vr, ph, f) : hlld(ul, vl, ur, vr, ph, f);
+    const bool ok = (SOLVER == 1) ? hllc(ul, vl, ur, vr, ph, f, dv) : hlld(ul, vl, ur, vr, ph, f, dv);
     // The solvers' own HLL fallback is an ordinary result of hllc_flux/hlld_flux:
     // it passes the finiteness test and gets the GLM overwrite like any other.
-    if (!ok) f = hll(ul, vl, flux(ul, vl, ph), ur, vr, flux(ur, vr, ph), ph);
+    if (!ok) f = hll(ul, vl, flux(ul, vl, ph), ur, vr, flux(ur, vr, ph), ph, dv);
     const double fbx = XDIR ? f.bn : f.bt;
     if (!finite(f.r) || !finite(f.e) || !finite(fbx) || !finite(f.psi))
-        return ok ? hll(ul, vl, flux(ul, vl, ph), ur, vr, flux(ur, vr, ph), ph) : f;
+        return ok ? hll(ul, vl, flux(ul, vl, ph), ur, vr, flux(ur, vr, ph), ph, dv) : f;
     glm_overwrite(f, ul, ur, ph);
     return f;
+}
+
+// On the device the face is evaluated with shared-reciprocal quotients first and again
+// with plain divisions only when some lane of the wave divided an operand outside
+// FastDiv's range; the flux is riemann_with<ExactDiv>'s bit for bit either way.
+template <int SOLVER, bool XDIR>
+MHD_DI Cons riemann(const Cons& ul, const Prim& vl, const Cons& ur, const Prim& vr, const Phys& ph) {
+#ifdef __HIP_DEVICE_COMPILE__
+    FastDiv fd;
+    const Cons f = riemann_with<SOLVER, XDIR>(ul, vl, ur, vr, ph, fd);
+    if (fd.wave_ok()) return f;
+#endif
+    ExactDiv ex;
+    return riemann_with<SOLVER, XDIR>(ul, vl, ur, vr, ph, ex);
 }
 
 // ---------------------------------------------------------------------------------
@@ -427,9 +597,13 @@ struct ReconRecord {
     bool pl, pr, pcl, pcr; // guards: W_L, W_R, then the half-stepped states
 };
 
-template <int LIMITER, class Record = RecordNone>
-MHD_DI void reconstruct(const Prim& wm, const Prim& wc, const Prim& wp, double hdtd, const Phys& ph, Face& lo,
-                        Face& hi, Record* rec = nullptr) {
+// dv.fast: whether to try CoreDiv where the operands allow it (never on the host).
+struct ReconDiv {
+    bool fast;
+};
+template <int LIMITER, class Record>
+MHD_DI void reconstruct_with(const Prim& wm, const Prim& wc, const Prim& wp, double hdtd, const Phys& ph, Face& lo,
+                             Face& hi, Record* rec, const ReconDiv& dv) {
     constexpr bool REC = !std::is_same<Record, RecordNone>::value;
     Prim wl, wr;
 #define MHD_SLOPE(k, f)                                       \
@@ -451,10 +625,24 @@ MHD_DI void reconstruct(const Prim& wm, const Prim& wc, const Prim& wp, double h
         if (!is_physical(wl)) wl = wc;
         if (!is_physical(wr)) wr = wc;
     }
-    const Cons ul = to_cons(wl, ph);
-    const Cons ur = to_cons(wr, ph);
-    const Prim vl = to_prim(ul, ph);
-    const Prim vr = to_prim(ur, ph);
+    Cons ul, ur;
+    Prim vl, vr;
+    // to_cons divides p by gamma - 1 and to_prim divides 1 by rho (to_cons keeps rho)
+    if (dv.fast && wave_in_range(ph.gm1, wl.p, wr.p, wl.r, wr.r)) {
+        CoreDiv cd;
+        const Rcp rg = cd.rcp(ph.gm1);
+        ul = to_cons(wl, ph, cd, rg);
+        ur = to_cons(wr, ph, cd, rg);
+        vl = to_prim(ul, ph, cd);
+        vr = to_prim(ur, ph, cd);
+    } else {
+        ExactDiv ex;
+        const Rcp rg = ex.rcp(ph.gm1);
+        ul = to_cons(wl, ph, ex, rg);
+        ur = to_cons(wr, ph, ex, rg);
+        vl = to_prim(ul, ph, ex);
+        vr = to_prim(ur, ph, ex);
+    }
     const Cons fl = flux(ul, vl, ph);
     const Cons fr = flux(ur, vr, ph);
     Cons cl, cr;
@@ -467,8 +655,16 @@ MHD_DI void reconstruct(const Prim& wm, const Prim& wc, const Prim& wp, double h
     MHD_HALF(r) MHD_HALF(mn) MHD_HALF(mt) MHD_HALF(mz) MHD_HALF(bn) MHD_HALF(bt) MHD_HALF(bz) MHD_HALF(e)
     MHD_HALF(psi)
 #undef MHD_HALF
-    const Prim pcl = to_prim(cl, ph);
-    const Prim pcr = to_prim(cr, ph);
+    Prim pcl, pcr;
+    if (dv.fast && wave_in_range(cl.r, cr.r)) {
+        CoreDiv cd;
+        pcl = to_prim(cl, ph, cd);
+        pcr = to_prim(cr, ph, cd);
+    } else {
+        ExactDiv ex;
+        pcl = to_prim(cl, ph, ex);
+        pcr = to_prim(cr, ph, ex);
+    }
     if constexpr (REC) {
         rec->pcl = is_physical(pcl);
         rec->pcr = is_physical(pcr);
@@ -480,6 +676,21 @@ MHD_DI void reconstruct(const Prim& wm, const Prim& wc, const Prim& wp, double h
         if (is_physical(pcl)) { lo.u = cl; lo.w = pcl; } else { lo.u = ul; lo.w = vl; }
         if (is_physical(pcr)) { hi.u = cr; hi.w = pcr; } else { hi.u = ur; hi.w = vr; }
     }
+}
+
+// The reconstruction's divisions by gamma - 1 and by rho: on the device, each group
+// whose operands every lane of the wave holds in FastDiv's range (one ballot, before
+// dividing) uses CoreDiv's sequence, any other group divides (the limiter keeps its
+// division); lo and hi are bit for bit the same either way, and their prims carry rr.
+template <int LIMITER, class Record = RecordNone>
+MHD_DI void reconstruct(const Prim& wm, const Prim& wc, const Prim& wp, double hdtd, const Phys& ph, Face& lo,
+                        Face& hi, Record* rec = nullptr) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const ReconDiv dv{true};
+#else
+    const ReconDiv dv{false};
+#endif
+    reconstruct_with<LIMITER>(wm, wc, wp, hdtd, ph, lo, hi, rec, dv);
 }
 
 // Conservative update with the non-finite revert (solver_cpu.cpp:386-392).
