@@ -44,7 +44,6 @@ struct Cons {
 struct Prim {
     double r, un, ut, uz, bn, bt, bz, p, psi;
     double b2;
-    double rr;  // on the device, the refined reciprocal of r that a / r forms (see FastDiv)
 };
 
 // Per-launch physics constants.
@@ -102,9 +101,7 @@ MHD_DI double refined_rcp(double b) {
 }
 
 struct ExactDiv {
-    // The reciprocal is formed for Prim::rr only; unused, the compiler drops it.
-    MHD_DI Rcp rcp(double b) { return Rcp{b, refined_rcp(b)}; }
-    MHD_DI Rcp given(double b, double r) { return Rcp{b, r}; }
+    MHD_DI Rcp rcp(double b) { return Rcp{b, 0.0}; }
     MHD_DI double q(double a, const Rcp& d) { return a / d.b; }
 };
 
@@ -122,11 +119,6 @@ struct FastDiv {
     MHD_DI Rcp rcp(double b) {
         note(b);
         return Rcp{b, refined_rcp(b)};
-    }
-    // A reciprocal formed earlier by refined_rcp(b) (Prim::rr).
-    MHD_DI Rcp given(double b, double r) {
-        note(b);
-        return Rcp{b, r};
     }
     MHD_DI double q(double a, const Rcp& d) {
         note(a);
@@ -151,7 +143,6 @@ struct FastDiv {
 // FastDiv's sequence for operands the caller has already found in range.
 struct CoreDiv {
     MHD_DI Rcp rcp(double b) { return Rcp{b, refined_rcp(b)}; }
-    MHD_DI Rcp given(double b, double r) { return Rcp{b, r}; }
     MHD_DI double q(double a, const Rcp& d) {
 #ifdef __HIP_DEVICE_COMPILE__
         const double q0 = a * d.r;
@@ -177,8 +168,7 @@ MHD_DI bool wave_in_range(double x0, double x1, double x2 = 0.0, double x3 = 0.0
 #endif
 }
 
-// physics.hpp:69-78.  rr keeps the refined reciprocal of rho for later quotients by
-// rho (fast_speed under FastDiv).
+// physics.hpp:69-78
 template <class D> MHD_DI Prim to_prim(const Cons& u, const Phys& ph, D& dv) {
     const Rcp r = dv.rcp(u.r);
     const double inv = dv.q(1.0, r);
@@ -195,7 +185,6 @@ template <class D> MHD_DI Prim to_prim(const Cons& u, const Phys& ph, D& dv) {
     w.p = ph.gm1 * (u.e - ke - 0.5 * b2);
     w.psi = u.psi;
     w.b2 = b2;
-    w.rr = r.r;
     return w;
 }
 MHD_DI Prim to_prim(const Cons& u, const Phys& ph) {
@@ -267,8 +256,8 @@ MHD_DI double sqrt_cr(double x) {
     return sqrt(x);
 }
 
-// physics.hpp:175-190 (fast magnetosonic speed along the normal); under FastDiv the
-// three divisions by rho use the reciprocal to_prim formed (w must come from to_prim).
+// physics.hpp:175-190 (fast magnetosonic speed along the normal); the three
+// divisions by rho share one reciprocal under FastDiv.
 template <class D> MHD_DI double fast_speed(const Prim& w, const Phys& ph, D& dv) {
     const Rcp r = dv.rcp(w.r);
     const double a2 = dv.q(ph.gamma * w.p, r);
@@ -681,7 +670,7 @@ MHD_DI void reconstruct_with(const Prim& wm, const Prim& wc, const Prim& wp, dou
 // The reconstruction's divisions by gamma - 1 and by rho: on the device, each group
 // whose operands every lane of the wave holds in FastDiv's range (one ballot, before
 // dividing) uses CoreDiv's sequence, any other group divides (the limiter keeps its
-// division); lo and hi are bit for bit the same either way, and their prims carry rr.
+// division); lo and hi are bit for bit the same either way.
 template <int LIMITER, class Record = RecordNone>
 MHD_DI void reconstruct(const Prim& wm, const Prim& wc, const Prim& wp, double hdtd, const Phys& ph, Face& lo,
                         Face& hi, Record* rec = nullptr) {

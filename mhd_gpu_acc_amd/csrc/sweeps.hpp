@@ -112,13 +112,11 @@ template <int WG, int NF = 9> struct Lds9 {
         v[0][t] = w.r; v[1][t] = w.un; v[2][t] = w.ut; v[3][t] = w.uz; v[4][t] = w.bn;
         v[5][t] = w.bt; v[6][t] = w.bz; v[7][t] = w.p; v[8][t] = w.psi;
         if constexpr (NF > 9) v[9][t] = w.b2;
-        if constexpr (NF > 10) v[10][t] = w.rr;
     }
     MHD_DI Prim getp(int t) const {
-        double b2 = 0.0, rr = 0.0;
+        double b2 = 0.0;
         if constexpr (NF > 9) b2 = v[9][t];
-        if constexpr (NF > 10) rr = v[10][t];
-        return Prim{v[0][t], v[1][t], v[2][t], v[3][t], v[4][t], v[5][t], v[6][t], v[7][t], v[8][t], b2, rr};
+        return Prim{v[0][t], v[1][t], v[2][t], v[3][t], v[4][t], v[5][t], v[6][t], v[7][t], v[8][t], b2};
     }
 };
 
@@ -133,7 +131,7 @@ template <int WG, int NF = 9> struct Lds9 {
 template <int WG> struct XLds {
     Lds9<WG> sw;      // primitives, later face fluxes
     Lds9<WG> su;      // lower interface states
-    Lds9<WG, 11> sv;  // their primitives (with |B|^2 and the reciprocal of rho)
+    Lds9<WG, 10> sv;  // their primitives
 };
 
 // One flattened segment: lane t holds cell k of the source grid (state u).
