@@ -58,7 +58,10 @@ __global__ __launch_bounds__(WG) MHD_X_ATTR void xsweep_kernel(const double* __r
     if (s >= end) return;
     int k = kb + s * (WG - 4);
     Cons un = src.load(min(k, last));  // clamp: tail lanes read valid memory
-    for (;;) {
+    // persistent blocks only: the one-segment blocks after them keep the default priority
+    const Progress<MHD_PRIO_X != 0> prog((end - s + gp - 1) / gp, looping);
+    for (int it = 0;; ++it) {
+        prog.at(it);
         const int sn = s + gp;
         const int kn = kb + sn * (WG - 4);
         const Cons u = un;  // the segment loaded one iteration ago
@@ -76,14 +79,14 @@ __global__ __launch_bounds__(256) MHD_Y_ATTR void ysweep_kernel(const double* __
                                                                 double* __restrict__ Un, Geom g, Phys ph,
                                                                 double dtd, double hdtd, double damp, int seg,
                                                                 double* __restrict__ frame,
-                                                                unsigned long long* __restrict__ acc) {
+                                                                unsigned long long* __restrict__ acc, int prio) {
     __shared__ double ring[YRING_DOUBLES * 256];
     const int i = NG + blockIdx.x * blockDim.x + threadIdx.x;
     const int j0 = NG + blockIdx.y * seg;
     const int j1 = min(j0 + seg, NG + g.ny);
     const double cand = ysweep_column<SOLVER, LIMITER, BUF>(Ut, Un, g, ph, dtd, hdtd, damp, frame, acc != nullptr, i,
                                                        i < NG + g.nx, j0, j1,
-                                                       YRing{ring, (int)threadIdx.x, (int)blockDim.x});
+                                                       YRing{ring, (int)threadIdx.x, (int)blockDim.x}, prio != 0);
     if (acc) wave_max_to(acc, cand);
 }
 
@@ -193,12 +196,16 @@ static hipError_t launch_step_sl(const StepArgs& a) {
     }
     const int ywg = (a.ywg == 64 || a.ywg == 128) ? a.ywg : 256;
     dim3 gy((unsigned)((g.nx + ywg - 1) / ywg), (unsigned)((g.ny + seg - 1) / seg));
+    // progress-ordered wave priority when the grid holds two or more rounds of waves at two
+    // per SIMD (8 per CU)
+    const long long ywaves = (long long)gy.x * gy.y * (ywg / 64);
+    const int prio = a.cus > 0 && ywaves >= 2LL * 8 * a.cus;
     if (buf)
         hipLaunchKernelGGL((ysweep_kernel<S, L, true>), gy, dim3(ywg), 0, a.stream, a.Utmp, a.Unew, g, ph, dtdy,
-                           0.5 * dtdy, a.damp, seg, a.frame, MHD_FUSED_CFL ? a.acc : nullptr);
+                           0.5 * dtdy, a.damp, seg, a.frame, MHD_FUSED_CFL ? a.acc : nullptr, prio);
     else
         hipLaunchKernelGGL((ysweep_kernel<S, L, false>), gy, dim3(ywg), 0, a.stream, a.Utmp, a.Unew, g, ph, dtdy,
-                           0.5 * dtdy, a.damp, seg, a.frame, MHD_FUSED_CFL ? a.acc : nullptr);
+                           0.5 * dtdy, a.damp, seg, a.frame, MHD_FUSED_CFL ? a.acc : nullptr, prio);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.ev && (e = hipEventRecord(a.ev[3], a.stream)) != hipSuccess) return e;
     const bool to_host = MHD_FUSED_CFL && a.acc && a.acc_host;

@@ -88,6 +88,40 @@ template <bool BUF, class V, class T> MHD_DI V plane_view(T* U, long long ps) {
     else return V(as_global(U), ps);
 }
 
+// A wave's issue priority ordered by its progress through its share of a sweep
+// (s_setprio, the wave's own state): 3 at the start, one step lower at each quarter.  Two
+// waves share each SIMD and the sequencer otherwise serves the older first, so the
+// younger falls behind and ends alone on its SIMD; ordered by progress, the waves on a
+// SIMD keep pace and finish closer together.  ON = false (or active = false, uniform)
+// leaves the priority alone.
+template <bool ON> struct Progress {
+    int q1, q2, q3;  // done counts at which the priority steps down (uniform)
+    bool active;
+    MHD_DI explicit Progress(int total, bool act = true)
+        : q1(total / 4), q2(total / 2), q3(3 * total / 4), active(act) {
+        if constexpr (ON) if (active) __builtin_amdgcn_s_setprio(3);
+    }
+    MHD_DI void at(int done) const {
+        if constexpr (ON) {
+            if (!active) return;
+            if (done == q1) __builtin_amdgcn_s_setprio(2);
+            else if (done == q2) __builtin_amdgcn_s_setprio(1);
+            else if (done == q3) __builtin_amdgcn_s_setprio(0);
+        }
+    }
+};
+// Knobs (compile time): progress-ordered priority in the y march where its launch asks for
+// it (MHD_PRIO_Y, default on; the plain launch asks when its grid holds two or more
+// rounds of waves: OT 4096^2 HLLD y sweep 0.874 -> 0.847 ms, while at 1024^2, one round,
+// it made the sweep 3 % slower) and in the persistent x sweep (MHD_PRIO_X, default off:
+// 1 % slower there).
+#ifndef MHD_PRIO_Y
+#define MHD_PRIO_Y 1
+#endif
+#ifndef MHD_PRIO_X
+#define MHD_PRIO_X 0
+#endif
+
 // Workgroup barrier that orders LDS only.  __syncthreads() fences global memory
 // too, which makes every lane wait for its in-flight global loads (the next
 // segment's prefetch) before the barrier.
@@ -247,7 +281,7 @@ struct YRing {
 template <int SOLVER, int LIMITER, bool BUF = false>
 MHD_DI double ysweep_column(const double* __restrict__ Ut, double* __restrict__ Un, const Geom& g, const Phys& ph,
                             double dtd, double hdtd, double damp, double* __restrict__ frame, bool want_cfl, int i,
-                            bool active, int j0, int j1, const YRing& ring) {
+                            bool active, int j0, int j1, const YRing& ring, bool prio = false) {
     using Src = std::conditional_t<BUF, BufPlanes<false>, Planes<false, GlobalT<const double>>>;
     using Dst = std::conditional_t<BUF, BufPlanes<false>, Planes<false, GlobalT<double>>>;
     const Src src = plane_view<BUF, Src>(Ut, g.ps);
@@ -265,7 +299,9 @@ MHD_DI double ysweep_column(const double* __restrict__ Ut, double* __restrict__ 
     // its stores and the CFL candidate then run while the load is in flight (the last
     // iteration loads row j1+1 again).
     Cons ur = src.load(j0 * g.tnx + ic);
+    const Progress<MHD_PRIO_Y != 0> prog(j1 + 2 - j0, prio);
     for (int r = j0; r <= j1 + 1; ++r) {
+        prog.at(r - j0);
         ring.put(slot, ur);
         const Prim w2 = to_prim(ur, ph);
         Face lo, hi;
