@@ -120,6 +120,12 @@ __global__ __launch_bounds__(WG) void batch_xsweep_kernel(const BatchRun* __rest
 // the tail of the launch.
 // Two waves per SIMD: HLLD otherwise takes 256 VGPRs plus a few AGPRs, which
 // halves the occupancy.
+// Knob (compile time): MHD_BATCH_PRIO = 1 (default) orders each wave's issue priority by its
+// progress through its current tile, as the plain y march does (64-run family 7.91 -> 8.00 G
+// cell-updates/s, interleaved runs on one box).
+#ifndef MHD_BATCH_PRIO
+#define MHD_BATCH_PRIO 1
+#endif
 template <int SOLVER, int LIMITER, bool BUF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void batch_ysweep_kernel(const BatchRun* __restrict__ runs,
                                                            const BatchStep* __restrict__ steps,
@@ -144,7 +150,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
         const int j0 = NG + tl.z;
         const int j1 = j0 + tl.w;
         const double cand = ysweep_column<SOLVER, LIMITER, BUF>(R.U[1], R.U[2 - P.cur], g, P.ph, P.dtdy, P.hdtdy,
-                                                           P.damp, R.frame, true, i, i < NG + g.nx, j0, j1, yr);
+                                                           P.damp, R.frame, true, i, i < NG + g.nx, j0, j1, yr,
+                                                           MHD_BATCH_PRIO != 0);
         wave_max_to(acc + tl.x, cand);
     }
 }
