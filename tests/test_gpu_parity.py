@@ -100,6 +100,34 @@ def test_orszag_tang_unit_vs_oracle(solver, limiter):
     assert_bitwise(r["U"], o["U"], "state")
 
 
+@pytest.mark.parametrize("variant", ["scaled", "tiny_bz"])
+@pytest.mark.parametrize("solver", [0, 1, 2, 3])
+@pytest.mark.parametrize("limiter", [0, 1])
+def test_quotient_range_fallbacks_vs_oracle(variant, solver, limiter):
+    """States whose division operands leave the range of the shared-reciprocal quotients
+    (frexp exponents beyond +-380): rho, momenta and E scaled by 1e120 everywhere (operands
+    near 2^399: every face and reconstruction takes the plain divisions), or a tiny B_z of
+    1e-150 (side-star numerators near 2^-498 in some lanes of a wave, in range elsewhere).
+    The run must still be the oracle's bit for bit."""
+    cfg = m.get_n_case_config("orszag_tang_unit", 1)
+    cfg.nx, cfg.ny = 96, 80
+    U0 = m.initial_state(4, cfg)
+    if variant == "scaled":
+        U0[:, :, [0, 1, 2, 3, 7]] *= 1e120
+        assert np.frexp(U0[20, 20, 0])[1] > 380
+    else:
+        x = np.arange(U0.shape[1])
+        U0[:, :, 6] = 1e-150 * np.sin(0.3 * x)[None, :]
+        assert np.frexp(1e-150 * 0.3)[1] < -380
+    say(f"[gpu] OT unit 96x80 {variant}, solver={solver} limiter={limiter} 10 steps")
+    r = gpu_run(cfg, U0, solver, limiter, 10)
+    o = oracle.run_loop(U0, ocase(cfg), solver, limiter, 10)
+    assert r["steps"] == o["nsteps"] == 10
+    assert_bitwise(r["dts"], o["dts"], "dt")
+    assert_bitwise(r["chs"], o["chs"], "c_h")
+    assert_bitwise(r["U"], o["U"], "state")
+
+
 def test_shock_cloud_vs_oracle():
     cfg = m.get_n_case_config("shock_cloud", 1)
     cfg.nx, cfg.ny = 160, 80
