@@ -118,6 +118,21 @@ def main():
     gpu_dts = np.concatenate([w["dts"], r["dts"]])
     run.close()
 
+    # The HBM bandwidth streaming kernels of known traffic reach on this device, in this run:
+    # the practical ceiling the sweeps' bytes per second are read against (a sweep reads and
+    # writes the same number of bytes, as the copy does).
+    probe_bytes = 1 << 31
+    pa = torch.zeros(probe_bytes // 8, dtype=torch.float64, device="cuda")
+    pb = torch.empty_like(pa)
+    torch.cuda.synchronize()
+    tbps = (C.c_double * 3)()
+    from mhd_gpu_acc_amd._lib import check
+    check(L.mhd_probe_bandwidth(pa.data_ptr(), pb.data_ptr(), probe_bytes, 10, tbps))
+    del pa, pb
+    hbm_probe = {"copy": tbps[0], "read": tbps[1], "write": tbps[2], "unit": "TB/s", "bytes": probe_bytes,
+                 "kernels": "16-byte lanes, 4 workgroups of 256 per CU, grid-stride; copy moves 2 x bytes "
+                            "(mhd_probe_bandwidth)"}
+
     out = {
         "metric": "cell-updates/s + %HBM peak, OT 4096² HLLD FP64; batched & adjoint vs plain",
         "value": value,
@@ -144,7 +159,9 @@ def main():
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL_SWEEP * cells,
                      "traffic_source": (f"profiles/pmc_ot{n}_{args.solver}_{args.limiter}.txt (rocprofv3 --pmc "
                                         "FETCH_SIZE / WRITE_SIZE runs of this kernel, copy-calibrated), not "
-                                        "measured in this run") if traffic else None},
+                                        "measured in this run") if traffic else None,
+                     "frac_of_measured_copy": achieved / (tbps[0] * 1e12)},
+        "hbm_probe": hbm_probe,
     }
 
     if not args.no_batch:

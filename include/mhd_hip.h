@@ -324,6 +324,15 @@ int mhd_drive(mhd_run* run, const mhd_drive_opts* opts, mhd_drive_result* result
 int mhd_profile(mhd_run* run, int enable);
 int mhd_profile_read(mhd_run* run, double* ms4, int* steps);
 
+/* The HBM bandwidth this device reaches on streaming kernels of known traffic (16-byte
+ * lanes, 4 workgroups of 256 lanes per CU, grid-stride), the practical ceiling against
+ * which the sweeps' bytes per second are read: tbps3[0] a copy of a into b (2 * bytes
+ * moved per launch), tbps3[1] a read of a, tbps3[2] a write of b, in TB/s, each timed over
+ * reps launches after one warm-up on the null stream (synchronous).  a and b are distinct
+ * device buffers of `bytes` bytes (a multiple of 16, at least 1 MiB); on return b holds a
+ * copy of a. */
+int mhd_probe_bandwidth(const double* a, double* b, long long bytes, int reps, double* tbps3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,7 @@ def lib():
     L.mhd_device_count.argtypes = [_ip]
     L.mhd_profile.argtypes = [C.c_void_p, C.c_int]
     L.mhd_profile_read.argtypes = [C.c_void_p, _dp, _ip]
+    L.mhd_probe_bandwidth.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, _dp]
     _lib = L
     return L
 

@@ -440,6 +440,42 @@ int mhd_profile_read(mhd_run* r, double* ms4, int* steps) {
     return MHD_OK;
 }
 
+int mhd_probe_bandwidth(const double* a, double* b, long long bytes, int reps, double* tbps3) {
+    if (!a || !b || !tbps3) return fail(MHD_ERR_INVALID, "null argument");
+    if (a == b || bytes < (1LL << 20) || bytes % 16 != 0 || reps < 1)
+        return fail(MHD_ERR_INVALID, "distinct buffers of at least 1 MiB (a multiple of 16 bytes) and reps >= 1");
+    int dev = 0, cus = 1;
+    MHD_HIP(hipGetDevice(&dev));
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    MHD_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(MHD_ERR_HIP, "hipEventCreate failed");
+    }
+    int rc = MHD_OK;
+    // write, read, then copy: b holds a copy of a on return
+    const int order[3] = {2, 1, 0};
+    for (int k : order) {
+        hipError_t e = mhd::launch_bandwidth(k, a, b, bytes, cus, nullptr);  // warm-up
+        if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+        for (int i = 0; i < reps && e == hipSuccess; ++i) e = mhd::launch_bandwidth(k, a, b, bytes, cus, nullptr);
+        if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+            rc = fail(MHD_ERR_HIP, hipGetErrorString(e));
+            break;
+        }
+        const double moved = (k == 0 ? 2.0 : 1.0) * (double)bytes;
+        tbps3[k] = moved * reps / (ms * 1e-3) / 1e12;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
 int mhd_sync(mhd_run* r) {
     if (!r) return fail(MHD_ERR_INVALID, "null run");
     MHD_HIP(hipStreamSynchronize(r->stream()));

@@ -136,6 +136,26 @@ __global__ __launch_bounds__(256) void max_b_kernel(const double* __restrict__ U
     wave_max_to(acc, m);
 }
 
+// ---------------------------------------------------------------------------------------
+// Streaming kernels of known traffic (mhd_probe_bandwidth): the HBM bandwidth the device
+// reaches for a copy, a read and a write, against which the sweeps' figures are read.
+// 16-byte lanes, grid-stride.
+// ---------------------------------------------------------------------------------------
+typedef double bw2 __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(256) void bw_copy_kernel(const bw2* __restrict__ a, bw2* __restrict__ b, long long n) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) b[i] = a[i];
+}
+__global__ __launch_bounds__(256) void bw_read_kernel(const bw2* __restrict__ a, double* __restrict__ sink, long long n) {
+    double s = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        s += a[i].x + a[i].y;
+    if (s != s) sink[0] = s;  // keeps the loads; written only for NaN data
+}
+__global__ __launch_bounds__(256) void bw_write_kernel(bw2* __restrict__ b, double v, long long n) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        b[i] = bw2{v, v};
+}
+
 }  // namespace mhd
 
 // ===========================================================================================
@@ -247,6 +267,18 @@ hipError_t launch_max_b(const double* U, int nx, int ny, long long ps, unsigned 
     long long nb = ((long long)nx * ny + 255) / 256;
     if (nb > 4096) nb = 4096;
     hipLaunchKernelGGL(max_b_kernel, dim3((unsigned)nb), dim3(256), 0, s, U, g, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_bandwidth(int kind, const double* a, double* b, long long bytes, int cus, hipStream_t s) {
+    const long long n = bytes / 16;
+    const dim3 grid((unsigned)(4 * (cus > 0 ? cus : 1)));  // 4 workgroups per CU: the best copy rate measured
+    if (kind == 0)
+        hipLaunchKernelGGL(bw_copy_kernel, grid, dim3(256), 0, s, (const bw2*)a, (bw2*)b, n);
+    else if (kind == 1)
+        hipLaunchKernelGGL(bw_read_kernel, grid, dim3(256), 0, s, (const bw2*)a, b, n);
+    else
+        hipLaunchKernelGGL(bw_write_kernel, grid, dim3(256), 0, s, (bw2*)b, 0.5, n);
     return hipGetLastError();
 }
 

@@ -65,6 +65,9 @@ bool fused_cfl();
 hipError_t launch_cfl(const double* U, int nx, int ny, long long ps, double gamma, double ch,
                       unsigned long long* acc, hipStream_t s);
 hipError_t launch_bc(double* U, const double* prev, int nx, int ny, long long ps, const int* bc, hipStream_t s);
+// Streaming kernel of known traffic on `bytes` (a multiple of 16) bytes: kind 0 copies a
+// into b, 1 reads a, 2 writes b.
+hipError_t launch_bandwidth(int kind, const double* a, double* b, long long bytes, int cus, hipStream_t s);
 // *acc (zeroed by the caller) gets the bits of max |B| over the interior cells of U.
 hipError_t launch_max_b(const double* U, int nx, int ny, long long ps, unsigned long long* acc, hipStream_t s);
 

@@ -368,3 +368,23 @@ def test_full_size_properties_4096():
     r = gpu_run(bw, U0, 2, 0, 20)
     rows = r["U"][2:-2, 2:-2]
     assert (rows == rows[:1]).all()
+
+
+def test_bandwidth_probe():
+    """mhd_probe_bandwidth: the copy leaves b equal to a, the three rates are positive
+    and below the 8 TB/s peak, and bad arguments are refused."""
+    import torch
+
+    from mhd_gpu_acc_amd._lib import check
+    L = m.lib()
+    n = 8 << 20  # doubles: 64 MiB
+    a = torch.arange(n, dtype=torch.float64, device="cuda")
+    b = torch.zeros_like(a)
+    torch.cuda.synchronize()
+    t = (C.c_double * 3)()
+    check(L.mhd_probe_bandwidth(a.data_ptr(), b.data_ptr(), 8 * n, 2, t))
+    say(f"[gpu] bandwidth probe on 64 MiB: copy {t[0]:.2f}, read {t[1]:.2f}, write {t[2]:.2f} TB/s")
+    assert torch.equal(a, b)
+    assert all(0.0 < t[k] < 8.0 for k in range(3))
+    assert L.mhd_probe_bandwidth(a.data_ptr(), a.data_ptr(), 8 * n, 2, t) != 0
+    assert L.mhd_probe_bandwidth(a.data_ptr(), b.data_ptr(), 1000, 2, t) != 0
