@@ -442,8 +442,11 @@ int mhd_profile_read(mhd_run* r, double* ms4, int* steps) {
 
 int mhd_probe_bandwidth(const double* a, double* b, long long bytes, int reps, double* tbps3) {
     if (!a || !b || !tbps3) return fail(MHD_ERR_INVALID, "null argument");
-    if (a == b || bytes < (1LL << 20) || bytes % 16 != 0 || reps < 1)
-        return fail(MHD_ERR_INVALID, "distinct buffers of at least 1 MiB (a multiple of 16 bytes) and reps >= 1");
+    const char* pa = reinterpret_cast<const char*>(a);
+    const char* pb = reinterpret_cast<const char*>(b);
+    const bool overlap = bytes > 0 && pa < pb + bytes && pb < pa + bytes;
+    if (overlap || bytes < (1LL << 20) || bytes % 16 != 0 || reps < 1)
+        return fail(MHD_ERR_INVALID, "disjoint buffers of at least 1 MiB (a multiple of 16 bytes) and reps >= 1");
     int dev = 0, cus = 1;
     MHD_HIP(hipGetDevice(&dev));
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;

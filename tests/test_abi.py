@@ -92,6 +92,13 @@ def test_errors_are_reported():
         m.get_n_case_config(42, 1)
     with pytest.raises(ValueError):
         m.get_n_case_config(2, 0)
+    # the bandwidth probe refuses bad arguments before touching a device: null, overlapping
+    # or short buffers, a size that is not a multiple of 16 bytes, reps < 1
+    L, t = m.lib(), (C.c_double * 3)()
+    base, mib = 1 << 40, 1 << 20
+    for a, b, nbytes, reps in ((0, base, mib, 1), (base, base + mib // 2, mib, 1), (base, base + 4 * mib, 1000, 1),
+                               (base, base + 4 * mib, mib + 8, 1), (base, base + 4 * mib, mib, 0)):
+        assert L.mhd_probe_bandwidth(a, b, nbytes, reps, t) == 1, (a, b, nbytes, reps)
 
 
 def test_no_device_is_an_error_not_a_fallback():
