@@ -28,6 +28,11 @@ LIMITERS = {"minmod": 0, "vanleer": 1}
 KERNELS = ("xsweep", "bc_after_x", "ysweep", "bc_after_y")
 
 
+def _progress(msg):
+    """One line on stderr per finished leg, so a long run shows where it is."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def read_traffic(n, solver, limiter):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled
     by a copy-kernel calibration), from profiles/pmc_*.txt when present."""
@@ -164,12 +169,16 @@ def main():
         "hbm_probe": hbm_probe,
     }
 
+    _progress(f"plain {n}^2 {args.solver} {args.limiter}: {out['ms_per_step']:.3f} ms/step")
     if not args.no_batch:
         out["batched"] = bench_batch(args.batch_steps)
+        _progress("batched form done")
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint()
+        _progress("adjoint form done")
     if not args.no_configs:
         out["configs"] = bench_configs(0.0 if args.no_cpu else args.config_cpu_seconds)
+        _progress("configurations done")
 
     if not args.no_cpu:
         import oracle
