@@ -26,6 +26,11 @@ BYTES_PER_CELL_STEP = 288   # two sweeps
 SOLVERS = {"hll": 0, "hllc": 1, "hlld": 2, "force": 3}
 LIMITERS = {"minmod": 0, "vanleer": 1}
 KERNELS = ("xsweep", "bc_after_x", "ysweep", "bc_after_y")
+# VALU lane-instructions per cell of the plain sweeps (rocprofv3 SQ_INSTS_VALU x 64 / cells,
+# profiles/valu_sweeps_hlld_vanleer.md) and the MI355X's VALU issue peak: 1024 SIMDs of 16
+# FP64 lanes, one wave64 instruction per 4 cycles, at the 2.4 GHz peak clock.
+VALU_PER_CELL = {("hlld", "vanleer"): {"ysweep": 1289, "xsweep": 1141}}
+VALU_PEAK = 1024 * 16 * 2.4e9  # lane-instructions/s
 
 
 def _progress(msg):
@@ -168,6 +173,17 @@ def main():
                      "frac_of_measured_copy": achieved / (tbps[0] * 1e12)},
         "hbm_probe": hbm_probe,
     }
+    per_cell = VALU_PER_CELL.get((args.solver, args.limiter), {}).get(dom)
+    if per_cell:
+        # The sweeps are bound by FP64 VALU issue, not by HBM: the same kernel against the
+        # VALU issue peak (instruction count from the committed counters, time from this run).
+        rate = per_cell * cells / (kern_ms[dom] * 1e-3)
+        out["valu_roofline"] = {"bound": "valu", "kernel": dom, "lane_instructions_per_cell": per_cell,
+                                "achieved": rate / 1e12, "peak": VALU_PEAK / 1e12, "unit": "T lane-instructions/s",
+                                "frac": rate / VALU_PEAK,
+                                "source": "profiles/valu_sweeps_hlld_vanleer.md (rocprofv3 SQ_INSTS_VALU x 64 / "
+                                          "cells); peak at 2.4 GHz, FP64 rcp/rsq (31 per cell) counted as one "
+                                          "issue each though they take four"}
 
     _progress(f"plain {n}^2 {args.solver} {args.limiter}: {out['ms_per_step']:.3f} ms/step")
     if not args.no_batch:
