@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) MHD_Y_ATTR void ysweep_kernel(const double* __
                                                                 double dtd, double hdtd, double damp, int seg,
                                                                 double* __restrict__ frame,
                                                                 unsigned long long* __restrict__ acc, int prio) {
-    __shared__ double ring[YRING_DOUBLES * 256];
+    extern __shared__ double ring[];  // YRING_DOUBLES doubles per lane of the block (set at launch)
     const int i = NG + blockIdx.x * blockDim.x + threadIdx.x;
     const int j0 = NG + blockIdx.y * seg;
     const int j1 = min(j0 + seg, NG + g.ny);
@@ -220,11 +220,12 @@ static hipError_t launch_step_sl(const StepArgs& a) {
     // per SIMD (8 per CU)
     const long long ywaves = (long long)gy.x * gy.y * (ywg / 64);
     const int prio = a.cus > 0 && ywaves >= 2LL * 8 * a.cus;
+    const size_t lds = (size_t)YRING_DOUBLES * ywg * sizeof(double);
     if (buf)
-        hipLaunchKernelGGL((ysweep_kernel<S, L, true>), gy, dim3(ywg), 0, a.stream, a.Utmp, a.Unew, g, ph, dtdy,
+        hipLaunchKernelGGL((ysweep_kernel<S, L, true>), gy, dim3(ywg), lds, a.stream, a.Utmp, a.Unew, g, ph, dtdy,
                            0.5 * dtdy, a.damp, seg, a.frame, MHD_FUSED_CFL ? a.acc : nullptr, prio);
     else
-        hipLaunchKernelGGL((ysweep_kernel<S, L, false>), gy, dim3(ywg), 0, a.stream, a.Utmp, a.Unew, g, ph, dtdy,
+        hipLaunchKernelGGL((ysweep_kernel<S, L, false>), gy, dim3(ywg), lds, a.stream, a.Utmp, a.Unew, g, ph, dtdy,
                            0.5 * dtdy, a.damp, seg, a.frame, MHD_FUSED_CFL ? a.acc : nullptr, prio);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.ev && (e = hipEventRecord(a.ev[3], a.stream)) != hipSuccess) return e;
