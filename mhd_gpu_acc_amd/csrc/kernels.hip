@@ -81,8 +81,12 @@ __global__ __launch_bounds__(256) MHD_Y_ATTR void ysweep_kernel(const double* __
                                                                 double* __restrict__ frame,
                                                                 unsigned long long* __restrict__ acc, int prio) {
     extern __shared__ double ring[];  // YRING_DOUBLES doubles per lane of the block (set at launch)
-    const int i = NG + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j0 = NG + blockIdx.y * seg;
+    // blockIdx.x: row segment, blockIdx.y: column block.  Workgroups go to the XCDs in turn
+    // by linear index, so each XCD marches segments of every column block; with the
+    // column block first it marched two column blocks only, and the work per column block
+    // differs with the flow (4096^2 HLLD y sweep 1 % slower).
+    const int i = NG + blockIdx.y * blockDim.x + threadIdx.x;
+    const int j0 = NG + blockIdx.x * seg;
     const int j1 = min(j0 + seg, NG + g.ny);
     const double cand = ysweep_column<SOLVER, LIMITER, BUF>(Ut, Un, g, ph, dtd, hdtd, damp, frame, acc != nullptr, i,
                                                        i < NG + g.nx, j0, j1,
@@ -215,7 +219,7 @@ static hipError_t launch_step_sl(const StepArgs& a) {
         seg = (int)(rows < 4 ? 4 : (rows > 64 ? 64 : rows));
     }
     const int ywg = (a.ywg == 64 || a.ywg == 128) ? a.ywg : 256;
-    dim3 gy((unsigned)((g.nx + ywg - 1) / ywg), (unsigned)((g.ny + seg - 1) / seg));
+    dim3 gy((unsigned)((g.ny + seg - 1) / seg), (unsigned)((g.nx + ywg - 1) / ywg));
     // progress-ordered wave priority when the grid holds two or more rounds of waves at two
     // per SIMD (8 per CU)
     const long long ywaves = (long long)gy.x * gy.y * (ywg / 64);
