@@ -213,6 +213,25 @@ def test_launch_shape_independence(knobs, monkeypatch):
     assert_bitwise(r["U"], o["U"], f"{knobs}")
 
 
+@pytest.mark.parametrize("knobs", [dict(MHD_YWG=64), dict(MHD_YWG=128), dict(MHD_YWG=64, MHD_YSEG=16),
+                                   dict(MHD_XWG=128, MHD_XGRID=2, MHD_XTAIL=150)])
+def test_launch_shapes_at_scale(knobs, monkeypatch):
+    """At 1024^2 the grids of both sweeps hold one or more full rounds of workgroups on the
+    chip (the 70x90 grid above holds a fraction of one): the narrow y workgroups, with their
+    LDS ring sized by the launch, and other x shapes give the default launch's bits."""
+    cfg = m.get_n_case_config("orszag_tang_unit", 1)
+    cfg.nx = cfg.ny = 1024
+    U0 = m.initial_state(4, cfg)
+    say("[gpu] OT 1024^2 default launch")
+    ref = gpu_run(cfg, U0, 2, 1, 6)
+    for k, v in knobs.items():
+        monkeypatch.setenv(k, str(v))
+    say(f"[gpu] OT 1024^2 {knobs}")
+    r = gpu_run(cfg, U0, 2, 1, 6)
+    assert_bitwise(r["dts"], ref["dts"], f"dt {knobs}")
+    assert_bitwise(r["U"], ref["U"], f"state {knobs}")
+
+
 def test_small_and_odd_grids():
     for nx, ny in ((2, 2), (3, 5), (5, 2), (65, 3), (257, 7)):
         cfg = m.get_n_case_config("orszag_tang_unit", 1)
