@@ -103,7 +103,7 @@ int dt_from_state(mhd_run* r, const double* U, bool reuse, double* dt) {
 }
 
 int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt, bool want_cfl, bool x_only,
-               hipStream_t stream) {
+               hipStream_t stream, bool fill_tmp) {
     mhd::StepArgs a{};
     a.Uold = Uold;
     a.Utmp = Utmp;
@@ -134,6 +134,7 @@ int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, doubl
     a.stream = stream ? stream : r->stream();
     a.ev = nullptr;
     a.x_only = x_only;
+    a.fill_tmp = fill_tmp;
     if (x_only) {
         a.acc = nullptr;
         a.acc_host = nullptr;
@@ -370,7 +371,8 @@ int mhd_compute_dt(mhd_run* r, double* dt) {
 
 int mhd_advance(mhd_run* r, double dt) {
     if (!r) return fail(MHD_ERR_INVALID, "null run");
-    if (int rc = step_grids(r, r->U[0], r->U[1], r->U[2], dt)) return rc;
+    // U* is the run's own: only the ghost rows the y sweep reads are needed
+    if (int rc = step_grids(r, r->U[0], r->U[1], r->U[2], dt, true, false, nullptr, false)) return rc;
     std::swap(r->U[0], r->U[2]);
     r->acc_valid = mhd::fused_cfl();
     return MHD_OK;

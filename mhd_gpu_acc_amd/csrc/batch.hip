@@ -4,9 +4,10 @@
 // Per step there is one launch per sweep for the whole batch -- a persistent grid
 // walking a tile list that spans every run (x: 252-cell flattened segments; y:
 // 64-column wave tiles of `seg` rows) -- and one launch per ghost fill.  The step
-// (parameter upload, x sweep, ghost fill, y sweep with fused damping and CFL
-// reduction, ghost fill, read-back of every run's CFL maximum) is captured once as
-// a hipGraph and replayed.
+// (parameter upload, x sweep, which also writes the ghost rows of U* its y sweep reads,
+// a ghost fill of U* only for runs with a Dirichlet bottom or top side, y sweep with
+// fused damping and CFL reduction, ghost fill, read-back of every run's CFL maximum) is
+// captured once as a hipGraph and replayed.
 //
 // Each run keeps its own c_h ratchet, dt and damping factor.  They are computed on
 // the host with exactly the arithmetic of mhd_compute_dt / mhd_advance (the damping
@@ -40,6 +41,7 @@ struct BatchRun {   // constant for the life of the batch
     long long ps;
     double gamma;
     int bc[4];
+    RowPush push;   // ghost rows of U* its x sweep writes (n = 0: filled by the phase-0 fill)
 };
 
 // Rewritten by the host before every step.  The step constants are computed on
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(WG) void batch_xsweep_kernel(const BatchRun* __rest
         using Dst = std::conditional_t<BUF, BufPlanes<true>, Planes<true, GlobalT<double>>>;
         const Dst dst = plane_view<BUF, Dst>(R.U[1], g.ps);
         const int k = NG * g.tnx - 2 + tl.y * (WG - 4) + t;
-        xsweep_segment<SOLVER, LIMITER, WG>(u, k, t, dst, g, ph, P.dtdx, P.hdtdx, L);
+        xsweep_segment<SOLVER, LIMITER, WG>(u, k, t, dst, g, ph, P.dtdx, P.hdtdx, L, R.push);
         if (qn >= end) break;
         lds_barrier();  // LDS is rewritten by the next tile
         q = qn;
@@ -156,10 +158,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
     }
 }
 
-// phase 0: ghosts of U* (previous grid = the state); phase 1: ghosts of the next
-// state (previous grid = U*, psi from the undamped frame), and block 0 hands the
-// runs' CFL maxima to the host: system-scope stores into pinned memory replace a
-// device-to-host copy after the step.
+// phase 0: ghosts of U* (previous grid = the state), for the runs whose x sweep does not
+// write the ghost rows their y sweep reads (a Dirichlet bottom or top side); phase 1: ghosts
+// of the next state (psi from the undamped frame), and block 0 hands the runs' CFL maxima
+// to the host: system-scope stores into pinned memory replace a device-to-host copy after
+// the step.  The next state's cells that the reference copies from the previous grid (U*,
+// through a Dirichlet side) are read from the state: U*'s fill copied each of them from the
+// state's cell at the same position.
 __global__ __launch_bounds__(256) void batch_bc_kernel(const BatchRun* __restrict__ runs,
                                                        const BatchStep* __restrict__ steps,
                                                        const int2* __restrict__ tiles, int phase,
@@ -183,7 +188,7 @@ __global__ __launch_bounds__(256) void batch_bc_kernel(const BatchRun* __restric
     const int t = tl.y * 256 + threadIdx.x;
     if (t >= ghost_count(g)) return;
     if (phase == 0) ghost_fill(R.U[1], R.U[P.cur], g, R.bc, nullptr, t);
-    else ghost_fill(R.U[2 - P.cur], R.U[1], g, R.bc, R.frame, t);
+    else ghost_fill(R.U[2 - P.cur], R.U[P.cur], g, R.bc, R.frame, t);
 }
 
 // compute_dt for states not produced by a batched y sweep.  Tile = (run, chunk).
@@ -236,7 +241,8 @@ struct mhd_batch {
     int2* d_xt = nullptr;  int nxt = 0;
     int4* d_yt = nullptr;  int nyt = 0;
     int2* d_bt = nullptr;  int nbt = 0;   // ghost cells of the next state (phase 1)
-    int2* d_bt0 = nullptr; int nbt0 = 0;  // ghost cells of U* the y sweep reads (phase 0)
+    int2* d_bt0 = nullptr; int nbt0 = 0;  // ghost cells of U* the y sweep reads (phase 0; runs
+                                          // with a Dirichlet bottom or top side only)
     int2* d_ct = nullptr;  int nct = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -280,12 +286,12 @@ int build_tiles(mhd_batch* b) {
         }
         const int ng = 2 * 2 * ny + 2 * 2 * tnx;
         for (int c = 0; c < (ng + 255) / 256; ++c) bt.push_back(make_int2(r, c));
-        // U* is read only by the y sweep (its ghost rows) and, through a Dirichlet side, by
-        // the next state's fill; without such a side its ghost columns (the first 2 NG ny
-        // ghost-cell numbers) are never read, so its fill starts at the chunk holding the
-        // first ghost-row cell.
-        const bool dirichlet = b->d[r].bc[0] == 3 || b->d[r].bc[1] == 3 || b->d[r].bc[2] == 3 || b->d[r].bc[3] == 3;
-        for (int c = dirichlet ? 0 : (2 * 2 * ny) / 256; c < (ng + 255) / 256; ++c) bt0.push_back(make_int2(r, c));
+        // U*'s ghost cells are read only by the y sweep (its ghost rows), which the x sweep
+        // writes itself unless the bottom or top side is Dirichlet; for those runs the fill
+        // starts at the chunk holding the first ghost-row cell (the ghost columns, the first
+        // 2 NG ny ghost-cell numbers, are never read).
+        if (b->runs[r].push.n == 0)
+            for (int c = (2 * 2 * ny) / 256; c < (ng + 255) / 256; ++c) bt0.push_back(make_int2(r, c));
         for (int c = 0; c < (nx * ny + 255) / 256; ++c) ct.push_back(make_int2(r, c));
     }
     std::stable_sort(yt.begin(), yt.end(), [](const int4& a, const int4& c) { return a.w > c.w; });
@@ -323,8 +329,9 @@ template <int S, int L> hipError_t launch_batch_step_sl(mhd_batch* b) {
         if (b->xwg == 128) launch_bx<S, L, 128, false>(b);
         else launch_bx<S, L, 256, false>(b);
     }
-    hipLaunchKernelGGL(batch_bc_kernel, dim3(b->nbt0), dim3(256), 0, b->stream, b->d_runs, b->d_steps, b->d_bt0, 0,
-                       (const unsigned long long*)nullptr, (unsigned long long*)nullptr, 0);
+    if (b->nbt0 > 0)
+        hipLaunchKernelGGL(batch_bc_kernel, dim3(b->nbt0), dim3(256), 0, b->stream, b->d_runs, b->d_steps, b->d_bt0, 0,
+                           (const unsigned long long*)nullptr, (unsigned long long*)nullptr, 0);
     if (b->buf) launch_by<S, L, true>(b);
     else launch_by<S, L, false>(b);
     hipLaunchKernelGGL(batch_bc_kernel, dim3(b->nbt), dim3(256), 0, b->stream, b->d_runs, b->d_steps, b->d_bt, 1,
@@ -353,8 +360,8 @@ void drop_graph(mhd_batch* b) {
     b->graph = nullptr;
 }
 
-// One step as a graph: parameter upload and the four launches; the last one stores
-// the CFL maxima into h_acc.
+// One step as a graph: parameter upload and the three or four launches; the last one
+// stores the CFL maxima into h_acc.
 int capture_graph(mhd_batch* b) {
     drop_graph(b);
     B_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
@@ -492,6 +499,7 @@ int mhd_batch_create(const mhd_run_desc* descs, int n, mhd_batch** out) {
         if (9ull * (unsigned long long)R.ps * 8 > BUF_MAX_BYTES) b->buf = false;
         R.gamma = d.gamma;
         std::memcpy(R.bc, d.bc, sizeof R.bc);
+        R.push = row_push(d.ny, d.bc[2], d.bc[3]);
         for (int k = 0; k < 3; ++k) {
             if (hipMalloc(&R.U[k], 9 * R.ps * sizeof(double)) != hipSuccess)
                 return bail(set_error(MHD_ERR_HIP, "hipMalloc(batch grid)"));

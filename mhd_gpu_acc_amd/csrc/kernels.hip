@@ -44,7 +44,8 @@ namespace mhd {
 template <int SOLVER, int LIMITER, int WG, bool BUF>
 __global__ __launch_bounds__(WG) MHD_X_ATTR void xsweep_kernel(const double* __restrict__ Uo, double* __restrict__ Ut, Geom g,
                                                     Phys ph, double dtd, double hdtd,
-                                                    unsigned long long* __restrict__ acc, int gp, int npers) {
+                                                    unsigned long long* __restrict__ acc, int gp, int npers,
+                                                    RowPush rp) {
     __shared__ XLds<WG> L;
     const int t = threadIdx.x;
     if (acc && blockIdx.x == 0 && t == 0) *acc = 0ull;  // CFL accumulator for this step's ysweep
@@ -66,7 +67,7 @@ __global__ __launch_bounds__(WG) MHD_X_ATTR void xsweep_kernel(const double* __r
         const int kn = kb + sn * (WG - 4);
         const Cons u = un;  // the segment loaded one iteration ago
         if (sn < end) un = src.load(min(kn, last));
-        xsweep_segment<SOLVER, LIMITER, WG>(u, k, t, dst, g, ph, dtd, hdtd, L);
+        xsweep_segment<SOLVER, LIMITER, WG>(u, k, t, dst, g, ph, dtd, hdtd, L, rp);
         if (sn >= end) break;
         lds_barrier();  // sw is rewritten by the next segment
         s = sn;
@@ -176,7 +177,7 @@ namespace mhd {
 bool fused_cfl() { return MHD_FUSED_CFL != 0; }
 
 template <int S, int L, int WG, bool BUF>
-static void launch_x(const StepArgs& a, const Geom& g, const Phys& ph, double dtdx) {
+static void launch_x(const StepArgs& a, const Geom& g, const Phys& ph, double dtdx, const RowPush& rp) {
     const long long flat = (long long)g.tnx * g.ny;
     const int nseg = (int)((flat + (WG - 4) - 1) / (WG - 4));
     // xgrid > 0: persistent grid of xgrid blocks per CU, plus xtail per mille of the
@@ -185,7 +186,7 @@ static void launch_x(const StepArgs& a, const Geom& g, const Phys& ph, double dt
     const int tail = a.xgrid > 0 ? (int)((long long)nseg * a.xtail / 1000) : 0;
     const int npers = nseg - tail;
     hipLaunchKernelGGL((xsweep_kernel<S, L, WG, BUF>), dim3((unsigned)(gp + tail)), dim3(WG), 0, a.stream, a.Uold,
-                       a.Utmp, g, ph, dtdx, 0.5 * dtdx, a.acc, gp, npers);
+                       a.Utmp, g, ph, dtdx, 0.5 * dtdx, a.acc, gp, npers, rp);
 }
 
 template <int S, int L>
@@ -196,17 +197,24 @@ static hipError_t launch_step_sl(const StepArgs& a) {
     hipError_t e;
     // Buffer-resource addressing for grids of at most 4 GiB (an OT 4096^2 grid is 1.2 GB).
     const bool buf = a.addr_buf && 9ull * (unsigned long long)a.ps * 8 <= BUF_MAX_BYTES;
+    // Utmp's ghost cells: filled by a launch of their own for callers that read Utmp (and
+    // across a Dirichlet bottom or top side, whose ghost rows come from Uold); otherwise the
+    // x sweep writes the ghost rows the y sweep reads itself.
+    const RowPush rp = (a.fill_tmp || a.x_only) ? RowPush{} : row_push(g.ny, a.bc[2], a.bc[3]);
     if (a.ev && (e = hipEventRecord(a.ev[0], a.stream)) != hipSuccess) return e;
-    if (a.xwg == 64) buf ? launch_x<S, L, 64, true>(a, g, ph, dtdx) : launch_x<S, L, 64, false>(a, g, ph, dtdx);
-    else if (a.xwg == 128) buf ? launch_x<S, L, 128, true>(a, g, ph, dtdx) : launch_x<S, L, 128, false>(a, g, ph, dtdx);
-    else buf ? launch_x<S, L, 256, true>(a, g, ph, dtdx) : launch_x<S, L, 256, false>(a, g, ph, dtdx);
+    if (a.xwg == 64) buf ? launch_x<S, L, 64, true>(a, g, ph, dtdx, rp) : launch_x<S, L, 64, false>(a, g, ph, dtdx, rp);
+    else if (a.xwg == 128)
+        buf ? launch_x<S, L, 128, true>(a, g, ph, dtdx, rp) : launch_x<S, L, 128, false>(a, g, ph, dtdx, rp);
+    else buf ? launch_x<S, L, 256, true>(a, g, ph, dtdx, rp) : launch_x<S, L, 256, false>(a, g, ph, dtdx, rp);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.ev && (e = hipEventRecord(a.ev[1], a.stream)) != hipSuccess) return e;
     const int nbc = 2 * NG * g.ny + 2 * NG * g.tnx;
-    hipLaunchKernelGGL(bc_kernel, dim3((nbc + 255) / 256), dim3(256), 0, a.stream, a.Utmp, a.Uold, g, a.bc[0],
-                       a.bc[1], a.bc[2], a.bc[3], (const double*)nullptr, (const unsigned long long*)nullptr,
-                       (unsigned long long*)nullptr, 0ull);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (rp.n == 0) {
+        hipLaunchKernelGGL(bc_kernel, dim3((nbc + 255) / 256), dim3(256), 0, a.stream, a.Utmp, a.Uold, g, a.bc[0],
+                           a.bc[1], a.bc[2], a.bc[3], (const double*)nullptr, (const unsigned long long*)nullptr,
+                           (unsigned long long*)nullptr, 0ull);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     if (a.x_only) return hipSuccess;
     if (a.ev && (e = hipEventRecord(a.ev[2], a.stream)) != hipSuccess) return e;
     // Rows per lane segment: enough column-waves x segments for 2 waves per SIMD
@@ -234,7 +242,11 @@ static hipError_t launch_step_sl(const StepArgs& a) {
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.ev && (e = hipEventRecord(a.ev[3], a.stream)) != hipSuccess) return e;
     const bool to_host = MHD_FUSED_CFL && a.acc && a.acc_host;
-    hipLaunchKernelGGL(bc_kernel, dim3((nbc + 255) / 256), dim3(256), 0, a.stream, a.Unew, a.Utmp, g, a.bc[0],
+    // Unew's ghost cells.  Those the reference copies from the previous grid (Utmp, through
+    // a Dirichlet side) are read from Uold: Utmp's fill copied each of them from Uold's cell
+    // at the same position, so the values are the same whether or not Utmp's ghost cells
+    // were filled.
+    hipLaunchKernelGGL(bc_kernel, dim3((nbc + 255) / 256), dim3(256), 0, a.stream, a.Unew, a.Uold, g, a.bc[0],
                        a.bc[1], a.bc[2], a.bc[3], (const double*)a.frame, (const unsigned long long*)a.acc,
                        to_host ? a.acc_host : (unsigned long long*)nullptr, a.seq);
     if ((e = hipGetLastError()) != hipSuccess) return e;

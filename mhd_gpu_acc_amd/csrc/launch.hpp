@@ -34,6 +34,8 @@ struct StepArgs {
     hipStream_t stream;
     hipEvent_t* ev;          // null, or 5 events recorded around the 4 launches
     bool x_only;             // only the x sweep and its ghost fill (Utmp); Unew is not formed
+    bool fill_tmp;           // fill all of Utmp's ghost cells (for callers that read Utmp); else
+                             // only the ghost rows the y sweep reads, written by the x sweep
 };
 
 hipError_t launch_step(const StepArgs& a);

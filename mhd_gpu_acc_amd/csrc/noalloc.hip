@@ -106,6 +106,7 @@ extern "C" int mhd_advance_grids_noalloc(const mhd_run_desc* d, double ch, const
     a.acc_host = nullptr;
     a.stream = stream;
     a.ev = nullptr;
+    a.fill_tmp = true;  // Utmp is the caller's: all of its ghost cells are filled, as the reference's are
     hipError_t e = mhd::launch_step(a);
     if (e != hipSuccess) return mhd::set_error(MHD_ERR_HIP, hipGetErrorString(e));
     if (max_speed) {

@@ -168,11 +168,23 @@ template <int WG> struct XLds {
     Lds9<WG, 10> sv;  // their primitives
 };
 
+// Ghost rows of U* that the x sweep writes itself: a cell of interior row src[e] is also
+// stored to ghost row dst[e] of its column (rho v and B_y negated when bit e of flip is
+// set), as the ghost fill copies it for an interior column (entries e < n).  The y sweep
+// reads only the interior columns of U*, so these rows are all of U*'s ghost cells it
+// needs.  n = 0 when a ghost fill of U* follows the sweep instead.
+struct RowPush {
+    int n;
+    int src[4], dst[4];
+    unsigned flip;
+};
+
 // One flattened segment: lane t holds cell k of the source grid (state u).
-// Lanes 2..WG-3 on an interior cell write the updated state to dst.
+// Lanes 2..WG-3 on an interior cell write the updated state to dst (and to the ghost
+// rows of rp that copy their row).
 template <int SOLVER, int LIMITER, int WG, class DstPlanes>
 MHD_DI void xsweep_segment(const Cons& u, int k, int t, const DstPlanes& dst, const Geom& g,
-                           const Phys& ph, double dtd, double hdtd, XLds<WG>& L) {
+                           const Phys& ph, double dtd, double hdtd, XLds<WG>& L, const RowPush& rp) {
     const Prim w = to_prim(u, ph);
     L.sw.put(t, w);
     lds_barrier();
@@ -192,8 +204,44 @@ MHD_DI void xsweep_segment(const Cons& u, int k, int t, const DstPlanes& dst, co
     if (t >= 2 && t < WG - 2 && k < g.tnx * g.tny) {
         const int i = k % g.tnx;
         const int j = k / g.tnx;
-        if (i >= NG && i < NG + g.nx && j < NG + g.ny) dst.store(k, update(u, dtd, f, L.sw.get(t - 1)));
+        if (i >= NG && i < NG + g.nx && j < NG + g.ny) {
+            const Cons n = update(u, dtd, f, L.sw.get(t - 1));
+            dst.store(k, n);
+            // every source row of rp lies within two rows of an interior edge
+            if (rp.n > 0 && (j < 2 * NG || j >= g.ny)) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (e < rp.n && j == rp.src[e]) {
+                        Cons v = n;
+                        if ((rp.flip >> e) & 1u) {  // x frame: m_t = rho v, B_t = B_y
+                            v.mt = -v.mt;
+                            v.bt = -v.bt;
+                        }
+                        dst.store(rp.dst[e] * g.tnx + i, v);
+                    }
+                }
+            }
+        }
     }
+}
+
+// The rows of rp for a grid whose bottom and top sides are of types bc_lo and bc_hi (each
+// ghost row's source row and reflection from map_axis); n = 0 when a side is Dirichlet,
+// whose ghost rows come from the previous grid.
+MHD_DI bool map_axis(int c, int n, int type_lo, int type_hi, int& s, bool& flip);
+MHD_DI RowPush row_push(int ny, int bc_lo, int bc_hi) {
+    RowPush rp{};
+    const int rows[4] = {0, 1, NG + ny, NG + ny + 1};
+    for (int e = 0; e < 4; ++e) {
+        int sj;
+        bool flip;
+        if (!map_axis(rows[e], ny, bc_lo, bc_hi, sj, flip)) return RowPush{};
+        rp.src[e] = sj;
+        rp.dst[e] = rows[e];
+        if (flip) rp.flip |= 1u << e;
+    }
+    rp.n = 4;
+    return rp;
 }
 
 // ---------------------------------------------------------------------------------------
