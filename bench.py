@@ -25,6 +25,9 @@ BYTES_PER_CELL_SWEEP = 144  # 9 FP64 read + 9 FP64 written per cell per directio
 BYTES_PER_CELL_STEP = 288   # two sweeps
 SOLVERS = {"hll": 0, "hllc": 1, "hlld": 2, "force": 3}
 LIMITERS = {"minmod": 0, "vanleer": 1}
+# Intervals between the HIP events a profiled step records: the x sweep, the gap to the y sweep
+# (holding U*'s ghost fill only where U* is read afterwards: a run's own steps skip it, the x
+# sweep writing the ghost rows the y sweep reads), the y sweep, the closing ghost fill.
 KERNELS = ("xsweep", "bc_after_x", "ysweep", "bc_after_y")
 # VALU lane-instructions per cell of the plain sweeps (rocprofv3 SQ_INSTS_VALU x 64 / cells,
 # profiles/valu_sweeps_hlld_vanleer.md) and the MI355X's VALU issue peak: 1024 SIMDs of 16
