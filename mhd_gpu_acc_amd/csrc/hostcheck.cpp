@@ -265,4 +265,26 @@ void hc_cfl_candidates(const double* U, int n, double gamma, double ch, double* 
     }
 }
 
+// The ghost rows the x sweep writes itself (row_push): out = {n, src[4], dst[4], flip}.
+void hc_row_push(int ny, int bc_lo, int bc_hi, int* out) {
+    const RowPush rp = row_push(ny, bc_lo, bc_hi);
+    out[0] = rp.n;
+    for (int e = 0; e < 4; ++e) {
+        out[1 + e] = rp.src[e];
+        out[5 + e] = rp.dst[e];
+    }
+    out[9] = (int)rp.flip;
+}
+
+// The closed-form source of ghost cell (gi, gj) (ghost_source): out = {i, j, prev, flip_x, flip_y}.
+void hc_ghost_source(int gi, int gj, int nx, int ny, const int* bc, int* out) {
+    const Geom g{nx, ny, nx + 2 * NG, ny + 2 * NG, (long long)(nx + 2 * NG) * (ny + 2 * NG)};
+    const Src s = ghost_source(gi, gj, g, bc);
+    out[0] = s.i;
+    out[1] = s.j;
+    out[2] = s.prev;
+    out[3] = s.flip_x;
+    out[4] = s.flip_y;
+}
+
 }  // extern "C"

@@ -31,6 +31,9 @@ def lib():
         L.hc_adjoint_step.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                       C.POINTER(C.c_int), _dp]
+        _ip = C.POINTER(C.c_int)
+        L.hc_row_push.argtypes = [C.c_int, C.c_int, C.c_int, _ip]
+        L.hc_ghost_source.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip]
         _lib = L
     return _lib
 
@@ -108,3 +111,18 @@ def cfl_candidates(U, gamma, ch):
     hit = np.zeros(n, dtype=np.int32)
     lib().hc_cfl_candidates(_p(U), n, gamma, ch, _p(fast), _p(exact), hit.ctypes.data_as(C.POINTER(C.c_int)))
     return fast, exact, hit.astype(bool)
+
+
+def row_push(ny, bc_lo, bc_hi):
+    """The ghost rows the x sweep writes itself: (n, src rows, dst rows, flip bits)."""
+    out = np.zeros(10, dtype=np.int32)
+    lib().hc_row_push(ny, bc_lo, bc_hi, out.ctypes.data_as(C.POINTER(C.c_int)))
+    return int(out[0]), [int(v) for v in out[1:5]], [int(v) for v in out[5:9]], int(out[9])
+
+
+def ghost_source(gi, gj, nx, ny, bc):
+    """The closed-form source of ghost cell (gi, gj): (i, j, prev, flip_x, flip_y)."""
+    b = np.ascontiguousarray(bc, dtype=np.int32)
+    out = np.zeros(5, dtype=np.int32)
+    lib().hc_ghost_source(gi, gj, nx, ny, b.ctypes.data_as(C.POINTER(C.c_int)), out.ctypes.data_as(C.POINTER(C.c_int)))
+    return int(out[0]), int(out[1]), bool(out[2]), bool(out[3]), bool(out[4])
