@@ -239,11 +239,12 @@ _IC_CACHE = {}
 
 
 def _time_plain(case_name, nx, ny, solver, limiter, warmup, steps, t_end=None, params=None, cfl=None,
-                cpu_seconds=0.0):
+                cpu_seconds=0.0, y_max=None):
     """Plain step of one configuration: warm-up, then `steps` timed steps (or the run to
     t_end when steps is None), with the sweep kernels timed by HIP events.  A run the
     scheme cannot finish (the reference's blow-up stop, dt below 1e-8 t_end) reports
-    its stop code and the time reached, as the reference driver does."""
+    its stop code and the time reached, as the reference driver does.  y_max replaces the
+    case's upper y bound (a slab of the case's domain)."""
     import ctypes as C
     import time
 
@@ -255,7 +256,9 @@ def _time_plain(case_name, nx, ny, solver, limiter, warmup, steps, t_end=None, p
         cfg.cfl = 0.8
     if cfl is not None:
         cfg.cfl = cfl
-    key = (case_name, nx, ny)
+    if y_max is not None:
+        cfg.y_max = y_max
+    key = (case_name, nx, ny, cfg.y_max)
     if key not in _IC_CACHE:
         _IC_CACHE.clear()
         _IC_CACHE[key] = m.initial_state(m.parse_case_id(case_name), cfg, params)
@@ -277,7 +280,8 @@ def _time_plain(case_name, nx, ny, solver, limiter, warmup, steps, t_end=None, p
         L.mhd_profile(run.handle, 0)
     k = max(nst.value, 1)
     ysweep = ms4[2] / k
-    out = {"workload": f"{case_name} {nx}x{ny}, {solver.upper()}, {limiter}, CFL {cfg.cfl}", "steps": r["steps"],
+    slab = f" on [{cfg.x_min:g},{cfg.x_max:g}]x[{cfg.y_min:g},{cfg.y_max:g}]" if y_max is not None else ""
+    out = {"workload": f"{case_name} {nx}x{ny}{slab}, {solver.upper()}, {limiter}, CFL {cfg.cfl}", "steps": r["steps"],
            "t": r["t"], "stop_code": r["code"], "value": r["steps"] * nx * ny / sec, "unit": "cell-updates/s",
            "ms_per_step": 1e3 * sec / max(r["steps"], 1), "xsweep_ms": ms4[0] / k, "ysweep_ms": ysweep,
            "ysweep_hbm_frac": BYTES_PER_CELL_SWEEP * nx * ny / (ysweep * 1e-3) / HBM_PEAK}
@@ -324,8 +328,9 @@ def _cpu_parity(U0, cfg, solver, limiter, t_end, budget):
 def bench_configs(cpu_seconds: float = 2.0):
     """The other plain configurations: Orszag-Tang at 4096^2 and 1024^2 with each Riemann
     solver (van Leer, CFL 0.8, 5 warm-up + 100 timed steps), the 1024^2 HLLD run to
-    t = 0.5 (CFL 0.4), the seeded shock-cloud 2048x1024 (HLLD, 200 steps) and Brio-Wu
-    800x4 (HLLD, minmod, CFL 0.8, to t = 0.1).  Each one also times the oracle on about
+    t = 0.5 (CFL 0.4), one 4096x512 y slab of the 4096^2 domain (HLLD, 100 steps), the
+    seeded shock-cloud 2048x1024 (HLLD, 200 steps) and Brio-Wu 800x4 (HLLD, minmod,
+    CFL 0.8, to t = 0.1).  Each one also times the oracle on about
     cpu_seconds of its first steps on the host's cores and reports the GPU's parity over
     those steps (cpu_value, rel_l1, dt_max_rel_dev; 0 skips that leg)."""
     import mhd_gpu_acc_amd as m
@@ -339,6 +344,9 @@ def bench_configs(cpu_seconds: float = 2.0):
     # of 0.4; at 0.8 its run (like the reference's) stops at the blow-up guard near t = 0.14
     res.append(_time_plain("orszag_tang_unit", 1024, 1024, "hlld", "vanleer", 0, None, t_end=0.5, cfl=0.4,
                            cpu_seconds=c))
+    # one GPU's share of the 4096^2 run split into 8 y slabs: 4096x512 cells of the same
+    # size (y in [0, 1/8], periodic), the shape each GPU of a slab decomposition steps
+    res.append(_time_plain("orszag_tang_unit", 4096, 512, "hlld", "vanleer", 5, 100, cpu_seconds=c, y_max=0.125))
     p = m.ICParams()
     p.seed = 20261203
     res.append(_time_plain("shock_cloud", 2048, 1024, "hlld", "vanleer", 5, 200, params=p, cpu_seconds=c))
