@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
                                                            unsigned long long* __restrict__ acc) {
     __shared__ double ring[YRING_DOUBLES * 256];
     const int lane = threadIdx.x & 63;
-    const YRing yr{ring, (int)threadIdx.x, (int)blockDim.x};
+    const YRing yr = YRing::of(ring, (int)threadIdx.x);
     for (;;) {
         // Every lane takes part (lane 0 adds 1, the others 0) and lane 0's ticket is
         // broadcast: no lane-divergent branch, so the exit test is uniform and the

@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) MHD_Y_ATTR void ysweep_kernel(const double* __
     const int j1 = min(j0 + seg, NG + g.ny);
     const double cand = ysweep_column<SOLVER, LIMITER, BUF>(Ut, Un, g, ph, dtd, hdtd, damp, frame, acc != nullptr, i,
                                                        i < NG + g.nx, j0, j1,
-                                                       YRing{ring, (int)threadIdx.x, (int)blockDim.x}, prio != 0);
+                                                       YRing::of(ring, (int)threadIdx.x), prio != 0);
     if (acc) wave_max_to(acc, cand);
 }
 

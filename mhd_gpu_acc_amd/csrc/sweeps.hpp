@@ -297,27 +297,32 @@ MHD_DI void wave_max_to(unsigned long long* acc, double m) {
 //   - the primitive of the row above the cell being reconstructed (the slope's lower
 //     neighbour), which is not needed while the face flux is formed: keeping it here
 //     instead of in registers leaves room for the next row's load to be in flight.
+// Each wave owns a region of YRING_DOUBLES x 64 doubles, so a field's lanes are 512 bytes
+// apart whatever the block's width: every access of a slot is an immediate offset from one
+// address (paired as ds_read2st64 / ds_write2st64).
 constexpr int YRING_DOUBLES = 36;  // per lane: 3 rows + 1 primitive, 9 doubles each
 struct YRing {
-    double* p;   // YRING_DOUBLES * n doubles of LDS
-    int t, n;    // this lane's index and the lane count of the block
+    double* b;  // this lane's first double: ring + wave * YRING_DOUBLES * 64 + lane
+    MHD_DI static YRing of(double* ring, int t) {
+        return YRing{ring + (t >> 6) * (YRING_DOUBLES * 64) + (t & 63)};
+    }
     MHD_DI void putw(const Prim& w) const {
-        double* q = p + 27 * n + t;
-        q[0] = w.r; q[n] = w.un; q[2 * n] = w.ut; q[3 * n] = w.uz; q[4 * n] = w.bn;
-        q[5 * n] = w.bt; q[6 * n] = w.bz; q[7 * n] = w.p; q[8 * n] = w.psi;
+        double* q = b + 27 * 64;
+        q[0] = w.r; q[64] = w.un; q[2 * 64] = w.ut; q[3 * 64] = w.uz; q[4 * 64] = w.bn;
+        q[5 * 64] = w.bt; q[6 * 64] = w.bz; q[7 * 64] = w.p; q[8 * 64] = w.psi;
     }
     MHD_DI Prim getw() const {
-        const double* q = p + 27 * n + t;
-        return Prim{q[0], q[n], q[2 * n], q[3 * n], q[4 * n], q[5 * n], q[6 * n], q[7 * n], q[8 * n], 0.0};
+        const double* q = b + 27 * 64;
+        return Prim{q[0], q[64], q[2 * 64], q[3 * 64], q[4 * 64], q[5 * 64], q[6 * 64], q[7 * 64], q[8 * 64], 0.0};
     }
     MHD_DI void put(int slot, const Cons& u) const {
-        double* q = p + slot * 9 * n + t;
-        q[0] = u.r; q[n] = u.mn; q[2 * n] = u.mt; q[3 * n] = u.mz; q[4 * n] = u.bn;
-        q[5 * n] = u.bt; q[6 * n] = u.bz; q[7 * n] = u.e; q[8 * n] = u.psi;
+        double* q = b + slot * (9 * 64);
+        q[0] = u.r; q[64] = u.mn; q[2 * 64] = u.mt; q[3 * 64] = u.mz; q[4 * 64] = u.bn;
+        q[5 * 64] = u.bt; q[6 * 64] = u.bz; q[7 * 64] = u.e; q[8 * 64] = u.psi;
     }
     MHD_DI Cons get(int slot) const {
-        const double* q = p + slot * 9 * n + t;
-        return Cons{q[0], q[n], q[2 * n], q[3 * n], q[4 * n], q[5 * n], q[6 * n], q[7 * n], q[8 * n]};
+        const double* q = b + slot * (9 * 64);
+        return Cons{q[0], q[64], q[2 * 64], q[3 * 64], q[4 * 64], q[5 * 64], q[6 * 64], q[7 * 64], q[8 * 64]};
     }
 };
 
