@@ -32,7 +32,7 @@ KERNELS = ("xsweep", "bc_after_x", "ysweep", "bc_after_y")
 # VALU lane-instructions per cell of the plain sweeps (rocprofv3 SQ_INSTS_VALU x 64 / cells,
 # profiles/valu_sweeps_hlld_vanleer.md) and the MI355X's VALU issue peak: 1024 SIMDs of 16
 # FP64 lanes, one wave64 instruction per 4 cycles, at the 2.4 GHz peak clock.
-VALU_PER_CELL = {("hlld", "vanleer"): {"ysweep": 1289, "xsweep": 1141}}
+VALU_PER_CELL = {("hlld", "vanleer"): {"ysweep": 1269, "xsweep": 1145}}
 VALU_PEAK = 1024 * 16 * 2.4e9  # lane-instructions/s
 
 
@@ -71,7 +71,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=40)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     ap.add_argument("--no-configs", action="store_true",
-                    help="skip the other plain configurations (all solvers at 4096^2 and 1024^2, shock-cloud, Brio-Wu)")
+                    help="skip the other plain configurations (all solvers at 4096^2 and 1024^2, a 4096x512 "
+                         "slab, shock-cloud, Brio-Wu)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="before the timed steps, copy BYTES between two device buffers once (a kernel "
                          "of known traffic for calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
