@@ -197,10 +197,10 @@ static hipError_t launch_step_sl(const StepArgs& a) {
     hipError_t e;
     // Buffer-resource addressing for grids of at most 4 GiB (an OT 4096^2 grid is 1.2 GB).
     const bool buf = a.addr_buf && 9ull * (unsigned long long)a.ps * 8 <= BUF_MAX_BYTES;
-    // Utmp's ghost cells: filled by a launch of their own for callers that read Utmp (and
-    // across a Dirichlet bottom or top side, whose ghost rows come from Uold); otherwise the
-    // x sweep writes the ghost rows the y sweep reads itself.
-    const RowPush rp = (a.fill_tmp || a.x_only) ? RowPush{} : row_push(g.ny, a.bc[2], a.bc[3]);
+    // Utmp's ghost cells: filled by a launch of their own for callers that read Utmp's ghost
+    // columns (and across a Dirichlet bottom or top side, whose ghost rows come from Uold);
+    // otherwise the x sweep writes the ghost rows the y sweep reads itself.
+    const RowPush rp = a.fill_tmp ? RowPush{} : row_push(g.ny, a.bc[2], a.bc[3]);
     if (a.ev && (e = hipEventRecord(a.ev[0], a.stream)) != hipSuccess) return e;
     if (a.xwg == 64) buf ? launch_x<S, L, 64, true>(a, g, ph, dtdx, rp) : launch_x<S, L, 64, false>(a, g, ph, dtdx, rp);
     else if (a.xwg == 128)

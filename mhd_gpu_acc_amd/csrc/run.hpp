@@ -51,8 +51,9 @@ int dt_from_state(mhd_run* r, const double* U, bool reuse, double* dt);
 // stream: where to launch (null: the run's stream; a replay on another stream records no
 // profiling events).  The launches use the run's frame buffer, so two steps of one run
 // must not be in flight on different streams at once.
-// fill_tmp: fill all of Utmp's ghost cells (callers that read Utmp afterwards: the tape's
-// replays, caller-owned grids); else only its ghost rows that the y sweep reads.
+// fill_tmp: fill all of Utmp's ghost cells (caller-owned grids, which may read any of
+// them); else only the interior columns of its ghost rows, which the y sweep and the
+// reverse y pass read (all of them through a Dirichlet bottom or top side).
 int step_grids(mhd_run* r, const double* Uold, double* Utmp, double* Unew, double dt, bool want_cfl = true,
                bool x_only = false, hipStream_t stream = nullptr, bool fill_tmp = true);
 

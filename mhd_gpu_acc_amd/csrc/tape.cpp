@@ -258,7 +258,7 @@ int mhd_tape_forward(mhd_tape* t, double* dts, double* chs) {
             const bool reuse = q == 0 ? r->acc_valid : mhd::fused_cfl();
             if (int rc = mhd::dt_from_state(r, S[q], reuse, &dt)) return rc;
             if (!std::isfinite(dt) || dt <= 0.0) return fail(MHD_ERR_DT, "dt non-finite or <= 0");
-            if (int rc = mhd::step_grids(r, S[q], T[q], S[q + 1], dt)) return rc;
+            if (int rc = mhd::step_grids(r, S[q], T[q], S[q + 1], dt, true, false, nullptr, false)) return rc;
         }
         t->dts[s] = dt;
         t->chs[s] = r->ch;
@@ -302,13 +302,15 @@ int mhd_tape_backward(mhd_tape* t, const double* lam_final, double* lam0, int la
         const int s0 = k * t->K, m = std::min(t->K, t->nsteps - s0);
         if (two && freed[b]) MHD_HIP(hipStreamWaitEvent(R, t->ev_free[b], 0));
         MHD_HIP(hipMemcpyAsync(S[0], t->ck[k], grid_bytes(r), hipMemcpyDeviceToDevice, R));
-        // The reverse steps read U and U* of each step; the state after the segment is
-        // only needed for the last segment's finiteness check, so elsewhere the last
-        // step stops after its x sweep.
+        // The reverse steps read U and U* of each step (U* at its interior columns only, so
+        // the x sweep's pushed ghost rows stand in for U*'s ghost fill where the bottom and
+        // top sides allow); the state after the segment is only needed for the last
+        // segment's finiteness check, so elsewhere the last step stops after its x sweep.
         for (int q = 0; q < m; ++q) {
             r->ch = t->chs[s0 + q];
             const bool x_only = q == m - 1 && k != t->nck - 1;
-            if (int e = mhd::step_grids(r, S[q], T[q], S[q + 1], t->dts[s0 + q], false, x_only, R)) return e;
+            if (int e = mhd::step_grids(r, S[q], T[q], S[q + 1], t->dts[s0 + q], false, x_only, R, false))
+                return e;
         }
         if (two) MHD_HIP(hipEventRecord(t->ev_ready[b], R));
         return MHD_OK;
