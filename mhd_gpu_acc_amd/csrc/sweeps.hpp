@@ -350,7 +350,10 @@ MHD_DI double ysweep_column(const double* __restrict__ Ut, double* __restrict__ 
     int slot = 0;  // ring slot of row r; row r-2 is in slot (slot + 1) % 3
     // Row r+1 is loaded during iteration r, once the face flux is formed: the update,
     // its stores and the CFL candidate then run while the load is in flight (the last
-    // iteration loads row j1+1 again).
+    // iteration loads row j1+1 again).  One load site for every iteration (the first has
+    // no face flux) keeps most of the loaded row in the registers the next iteration
+    // reads: two sites left it to a copy at the end of every iteration, and 13 more SGPRs
+    // spilled.
     Cons ur = src.load(j0 * g.tnx + ic);
     const Progress<MHD_PRIO_Y != 0> prog(j1 + 2 - j0, prio);
     for (int r = j0; r <= j1 + 1; ++r) {
@@ -360,10 +363,10 @@ MHD_DI double ysweep_column(const double* __restrict__ Ut, double* __restrict__ 
         Face lo, hi;
         reconstruct<LIMITER>(ring.getw(), w1, w2, hdtd, ph, lo, hi);  // cell r-1
         ring.putw(w1);
-        if (r < j0 + 1) ur = src.load((r + 1) * g.tnx + ic);
+        Cons f;  // formed from the second iteration on
+        if (r >= j0 + 1) f = riemann<SOLVER, false>(hu, hw, lo.u, lo.w, ph);  // face (r-2 | r-1)
+        ur = src.load((r < j1 + 1 ? r + 1 : j1 + 1) * g.tnx + ic);
         if (r >= j0 + 1) {
-            const Cons f = riemann<SOLVER, false>(hu, hw, lo.u, lo.w, ph);  // face (r-2 | r-1)
-            ur = src.load((r < j1 + 1 ? r + 1 : j1 + 1) * g.tnx + ic);
             if (r >= j0 + 2) {
                 const int j = r - 2;
                 const int k = j * g.tnx + ic;
