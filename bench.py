@@ -32,7 +32,7 @@ KERNELS = ("xsweep", "bc_after_x", "ysweep", "bc_after_y")
 # VALU lane-instructions per cell of the plain sweeps (rocprofv3 SQ_INSTS_VALU x 64 / cells,
 # profiles/valu_sweeps_hlld_vanleer.md) and the MI355X's VALU issue peak: 1024 SIMDs of 16
 # FP64 lanes, one wave64 instruction per 4 cycles, at the 2.4 GHz peak clock.
-VALU_PER_CELL = {("hlld", "vanleer"): {"ysweep": 1269, "xsweep": 1145}}
+VALU_PER_CELL = {("hlld", "vanleer"): {"ysweep": 1264, "xsweep": 1145}}
 VALU_PEAK = 1024 * 16 * 2.4e9  # lane-instructions/s
 
 
